@@ -1,0 +1,144 @@
+/*
+ * libssq — MI355X (gfx950) persistent-homology reduction engine, C ABI.
+ *
+ * Drop-in boundary for the SpecSeq++ reduction path. Every entry point takes plain
+ * pointers and sizes, returns an int status (never exits), and names the reference
+ * interface it replaces (paths relative to the SpecSeq++ tree).
+ *
+ * Index semantics: boundary indices and pair indices are the file's 0-based column
+ * indices for both ASCII and binary input (the reference's internal +1 shift and dummy
+ * column 0, main.cu:102-103,115, are not part of the contract).
+ */
+#ifndef SSQ_H
+#define SSQ_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (replace exit(EXIT_FAILURE) in kernel.h:26-32 and the ignored read error
+ * at main.cu:329). */
+#define SSQ_OK 0
+#define SSQ_EIO 1      /* file cannot be opened / read / written                       */
+#define SSQ_EFORMAT 2  /* malformed PHAT file, index out of range, bad edge column      */
+#define SSQ_ENOMEM 3   /* device or host memory exhausted (never silently degraded)   */
+#define SSQ_EDEVICE 4  /* HIP runtime error                                            */
+#define SSQ_EINVAL 5   /* invalid argument / parameter                                 */
+#define SSQ_EBUSY 6    /* server queue full (retry later)                              */
+
+/* Host CSC boundary matrix. Rows of each column are sorted, unique, in [0, n). */
+typedef struct ssq_matrix {
+  int64_t n;        /* number of columns (cells)            */
+  int64_t nnz;      /* number of nonzeros                   */
+  int64_t* col_ptr; /* n+1 offsets into rows                */
+  int32_t* rows;    /* nnz row indices                      */
+  int32_t* dims;    /* n cell dimensions                    */
+} ssq_matrix;
+
+/* The 10 numeric arguments of the specseq++ command line (main.cu:285-294, README.md:26).
+ * level1/level2 are the slot capacities of the two-level column store; level 3 slots hold
+ * 8*level2 entries (specseq.h:149-154). */
+typedef struct ssq_params {
+  int32_t dualize;       /* 0/1: reduce the anti-transposed matrix (cohomology)          */
+  int32_t model;         /* bit0: high-dimension guided clearing, bit1: edge collapse    */
+  int32_t max_iteration; /* neighbours examined per max-dim cell by guided clearing      */
+  int32_t ss_block_size; /* first page width (columns and rows)                         */
+  float growth;          /* page growth factor (>= 1)                                    */
+  int32_t queue_size;    /* additions in flight per round (one wavefront each)          */
+  int32_t level1;        /* level-1 slot capacity (int32 entries)                        */
+  int32_t level2;        /* level-2 slot capacity (int32 entries)                        */
+} ssq_params;
+
+/* Persistence pairs, sorted by birth; optional essential (never-paired) cells. */
+typedef struct ssq_pairs {
+  int64_t np;
+  int64_t* pairs; /* 2*np: birth, death */
+  int64_t ne;
+  int64_t* essential; /* ne cell indices, ascending */
+} ssq_pairs;
+
+/* Per-call statistics (the reference prints these; specseq.h:291-296,576-579,1053). */
+typedef struct ssq_stats {
+  double t_init_ms;      /* pre-passes: collapse, guided clearing, init passes          */
+  double t_reduce_ms;    /* page loop over all dimensions                               */
+  double t_total_ms;     /* whole call on the device side                               */
+  double t_collapse_ms;  /* host part of the edge collapse (model bit 1)                */
+  int64_t n_pending;     /* columns left for the page loop after the pre-passes        */
+  int64_t n_apparent;    /* apparent pairs found by the init passes                    */
+  int64_t n_cleared_hd;  /* max-dim cells cleared by guided clearing                   */
+  int64_t n_kept_edges;  /* edges kept by the collapse                                 */
+  int64_t n_pages;       /* pages run                                                  */
+  int64_t n_rounds;      /* claim/add rounds run                                       */
+  int64_t n_additions;   /* column additions performed                                 */
+  double add_bytes;      /* algorithmic bytes of those additions: 4(|a|+|b|+|a^b|)+24 */
+  int64_t store_bytes;   /* device bytes reserved for the column store                 */
+} ssq_stats;
+
+/* ---- I/O (main.cu:90-167 processFiles; main.cu:191-229 writeVectorToFile) ---------- */
+int ssq_load_phat(const char* path, int binary, ssq_matrix* out);
+int ssq_save_pairs(const char* path, int binary, const ssq_pairs* pairs);
+void ssq_matrix_free(ssq_matrix* m);
+void ssq_pairs_free(ssq_pairs* p);
+
+/* Default parameters (README.md:44-53: 102400, 1.05, 65536, level1/level2 1000/2000). */
+void ssq_default_params(ssq_params* p);
+
+/* ---- Engine handle: one per host thread; owns a HIP stream and its device state. ---- */
+typedef struct ssq_handle ssq_handle;
+int ssq_create(int device, ssq_handle** out);
+int ssq_destroy(ssq_handle* h);
+const char* ssq_last_error(const ssq_handle* h);
+
+/* Whole reduction of one host matrix: replaces the SpecSeqPlusPlus constructor
+ * (specseq.h:486-512), GPU_Init (513-590), GPU_Compute (592-1060) and
+ * GetPersistencePairs (1182-1193).  stats may be NULL. */
+int ssq_reduce(ssq_handle* h, const ssq_matrix* m, const ssq_params* p, ssq_pairs* out,
+               ssq_stats* stats);
+
+/* Same, with the matrix already resident in device memory (device pointers d_col_ptr
+ * [n+1], d_rows[nnz], d_dims[n]).  Pairs are written to device memory d_pairs_out (2*n
+ * int64 capacity) and their count to *np_out.  Used by the timed region of bench.py. */
+int ssq_reduce_device(ssq_handle* h, int64_t n, int64_t nnz, const int64_t* d_col_ptr,
+                      const int32_t* d_rows, const int32_t* d_dims, const ssq_params* p,
+                      int64_t* d_pairs_out, int64_t* np_out, ssq_stats* stats);
+
+/* ---- Batching (new; no reference counterpart): many complexes, one engine pass. ---- */
+/* The matrices are laid out block-diagonally on the device and reduced together; outs[i]
+ * receives complex i's pairs in its own indices. */
+int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, int count,
+                     const ssq_params* p, ssq_pairs* outs, ssq_stats* stats);
+
+/* ---- Serving (new): continuous batching request loop on one GPU. -------------------- */
+typedef struct ssq_server ssq_server;
+typedef struct ssq_server_config {
+  int device;
+  int streams;               /* batches that may be on the device at once (>=1)        */
+  int max_batch;             /* requests per batch                                     */
+  int64_t mem_budget_bytes;  /* admission budget of estimated device bytes (0 = auto) */
+  ssq_params params;
+} ssq_server_config;
+int ssq_server_create(const ssq_server_config* cfg, ssq_server** out);
+/* Submit a matrix (copied) or a point cloud (npts x pdim float32, Rips threshold r,
+ * max dimension maxdim).  *ticket receives the request id. */
+int ssq_server_submit_matrix(ssq_server* s, const ssq_matrix* m, int64_t* ticket);
+int ssq_server_submit_points(ssq_server* s, const float* pts, int64_t npts, int pdim,
+                             float r, int maxdim, int64_t* ticket);
+/* Non-blocking poll: returns SSQ_OK and fills out (caller frees) when ticket is done,
+ * SSQ_EBUSY when still pending.  latency_ms (may be NULL): submit-to-completion. */
+int ssq_server_poll(ssq_server* s, int64_t ticket, ssq_pairs* out, double* latency_ms);
+int ssq_server_wait_all(ssq_server* s);
+int ssq_server_destroy(ssq_server* s);
+
+/* ---- Inputs (phat-highclear/addons/rips.cpp:49-87 semantics) ------------------------- */
+/* Vietoris-Rips boundary matrix of npts points in R^pdim: edges with float32 distance
+ * <= r, cliques up to dimension maxdim, ordered by (filtration, dim, lexicographic vertex
+ * ids) with filtration = longest edge. */
+int ssq_vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim,
+                 ssq_matrix* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

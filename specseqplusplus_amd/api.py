@@ -1,0 +1,145 @@
+"""Python host API over libssq: matrices, reduction handles, batches and the server."""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Matrix:
+    """CSC boundary matrix: colptr int64[n+1], rows int32[nnz] (sorted per column), dims int32[n]."""
+
+    def __init__(self, colptr, rows, dims):
+        self.colptr = np.ascontiguousarray(colptr, dtype=np.int64)
+        self.rows = np.ascontiguousarray(rows, dtype=np.int32)
+        self.dims = np.ascontiguousarray(dims, dtype=np.int32)
+
+    @property
+    def n(self):
+        return int(len(self.dims))
+
+    @property
+    def nnz(self):
+        return int(len(self.rows))
+
+    def c_struct(self):
+        m = L.ssq_matrix()
+        m.n = self.n
+        m.nnz = self.nnz
+        m.col_ptr = self.colptr.ctypes.data_as(L.i64p)
+        m.rows = self.rows.ctypes.data_as(L.i32p) if self.nnz else ctypes.cast(None, L.i32p)
+        m.dims = self.dims.ctypes.data_as(L.i32p)
+        return m
+
+    @classmethod
+    def from_c(cls, m):
+        n, nnz = m.n, m.nnz
+        colptr = np.ctypeslib.as_array(m.col_ptr, shape=(n + 1,)).copy()
+        rows = np.ctypeslib.as_array(m.rows, shape=(max(nnz, 1),))[:nnz].copy()
+        dims = np.ctypeslib.as_array(m.dims, shape=(max(n, 1),))[:n].copy()
+        return cls(colptr, rows, dims)
+
+
+def params(dualize=0, model=0, max_iteration=10, ss_block_size=102400, growth=1.05, queue_size=65536,
+           level1=1000, level2=2000):
+    p = L.ssq_params()
+    p.dualize, p.model, p.max_iteration, p.ss_block_size = dualize, model, max_iteration, ss_block_size
+    p.growth, p.queue_size, p.level1, p.level2 = growth, queue_size, level1, level2
+    return p
+
+
+def _pairs_from_c(pr):
+    np_ = pr.np
+    pairs = np.ctypeslib.as_array(pr.pairs, shape=(max(2 * np_, 1),))[:2 * np_].copy().reshape(-1, 2)
+    ess = np.ctypeslib.as_array(pr.essential, shape=(max(pr.ne, 1),))[:pr.ne].copy()
+    L.lib().ssq_pairs_free(ctypes.byref(pr))
+    return pairs, ess
+
+
+def load_phat(path, binary=False):
+    m = L.ssq_matrix()
+    L.check(L.lib().ssq_load_phat(path.encode(), int(binary), ctypes.byref(m)), path)
+    out = Matrix.from_c(m)
+    L.lib().ssq_matrix_free(ctypes.byref(m))
+    return out
+
+
+def save_pairs(path, pairs, binary=False):
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1)
+    pr = L.ssq_pairs()
+    pr.np = len(pairs) // 2
+    pr.pairs = pairs.ctypes.data_as(L.i64p)
+    pr.ne = 0
+    L.check(L.lib().ssq_save_pairs(path.encode(), int(binary), ctypes.byref(pr)), path)
+
+
+def vr_build(points, r, maxdim):
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    m = L.ssq_matrix()
+    L.check(L.lib().ssq_vr_build(pts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), pts.shape[0], pts.shape[1],
+                                 float(r), int(maxdim), ctypes.byref(m)))
+    out = Matrix.from_c(m)
+    L.lib().ssq_matrix_free(ctypes.byref(m))
+    return out
+
+
+class Handle:
+    """One engine (HIP stream + device state) on one GPU."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().ssq_create(int(device), ctypes.byref(self._h)), "ssq_create")
+
+    def close(self):
+        if self._h:
+            L.lib().ssq_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self):
+        return L.lib().ssq_last_error(self._h).decode()
+
+    def reduce(self, m, p=None, stats=False):
+        p = p or params()
+        pr = L.ssq_pairs()
+        st = L.ssq_stats()
+        ms = m.c_struct()
+        L.check(L.lib().ssq_reduce(self._h, ctypes.byref(ms), ctypes.byref(p), ctypes.byref(pr), ctypes.byref(st)),
+                self.error())
+        pairs, ess = _pairs_from_c(pr)
+        return (pairs, ess, st.as_dict()) if stats else (pairs, ess)
+
+    def reduce_device(self, n, nnz, d_colptr, d_rows, d_dims, d_pairs, p=None):
+        """Device-resident reduction: arguments are device pointers (ints)."""
+        p = p or params()
+        np_ = ctypes.c_int64()
+        st = L.ssq_stats()
+        L.check(L.lib().ssq_reduce_device(self._h, n, nnz, d_colptr, d_rows, d_dims, ctypes.byref(p), d_pairs,
+                                          ctypes.byref(np_), ctypes.byref(st)), self.error())
+        return np_.value, st.as_dict()
+
+    def batch_reduce(self, mats, p=None, stats=False):
+        p = p or params()
+        structs = [m.c_struct() for m in mats]
+        arr = (ctypes.POINTER(L.ssq_matrix) * len(mats))(*[ctypes.pointer(s) for s in structs])
+        outs = (L.ssq_pairs * len(mats))()
+        st = L.ssq_stats()
+        L.check(L.lib().ssq_batch_reduce(self._h, arr, len(mats), ctypes.byref(p), outs, ctypes.byref(st)),
+                self.error())
+        res = [_pairs_from_c(outs[i]) for i in range(len(mats))]
+        return (res, st.as_dict()) if stats else res
+
+
+_default = None
+
+
+def reduce(m, p=None, stats=False):
+    global _default
+    if _default is None:
+        _default = Handle(0)
+    return _default.reduce(m, p, stats)

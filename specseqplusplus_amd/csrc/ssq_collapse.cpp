@@ -1,0 +1,175 @@
+// Flag-complex edge collapse (model bit 1) — host part of libssq.
+//
+// Produces exactly the kept-edge set of GUDHI's sequential collapser as SpecSeq++ calls
+// it (specseq.h:209-212,410-424 -> Flag_complex_edge_collapser.h:416-482; edge i has
+// filtration i).  The algorithm is split in two:
+//
+//  1. Main-loop decisions.  When the sequential loop tests edge i, every critical edge has
+//     index < i, so "valid" (:148-152) reduces to "index < i": the test is a pure function
+//     of the graph of edges < i.  These tests are independent and run on all host threads.
+//  2. Cascades (set_edge_critical, :326-359).  For each edge that the main loop finds
+//     critical, in index order, the non-critical edges of its 3-cliques are re-tested in
+//     decreasing index order with the critical set of that moment.  This part is
+//     inherently sequential and runs on one thread.
+//
+// The domination predicate (:224-303) is: with I the common valid neighbours of (a,b),
+// the edge is dominated iff |I| == 1 or some w in I has I inside its closed valid
+// neighbourhood.  It does not depend on vertex labels, so vertices are compacted.
+#include <algorithm>
+#include <atomic>
+#include <queue>
+#include <thread>
+#include <vector>
+
+#include "ssq_internal.h"
+
+namespace {
+
+struct Graph {
+  std::vector<int64_t> off;
+  std::vector<int32_t> nbr, eid;
+  std::vector<uint8_t> crit;
+};
+
+template <typename Valid>
+int64_t common(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* out, int32_t* ea,
+               int32_t* eb) {
+  int64_t i = g.off[a], ie = g.off[a + 1], j = g.off[b], je = g.off[b + 1], k = 0;
+  while (i < ie && j < je) {
+    if (!valid(g.eid[i])) { i++; continue; }
+    if (!valid(g.eid[j])) { j++; continue; }
+    int32_t x = g.nbr[i], y = g.nbr[j];
+    if (x < y) i++;
+    else if (x > y) j++;
+    else {
+      if (ea) { ea[k] = g.eid[i]; eb[k] = g.eid[j]; }
+      out[k++] = x;
+      i++;
+      j++;
+    }
+  }
+  return k;
+}
+
+template <typename Valid>
+bool dominated(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* I) {
+  int64_t m = common(g, a, b, valid, I, nullptr, nullptr);
+  if (m <= 1) return m == 1;
+  for (int64_t c = 0; c < m; c++) {
+    int32_t w = I[c];
+    int64_t p = g.off[w], pe = g.off[w + 1];
+    bool ok = true;
+    for (int64_t t = 0; t < m && ok; t++) {
+      int32_t x = I[t];
+      if (x == w) continue;
+      while (p < pe && g.nbr[p] < x) p++;
+      ok = p < pe && g.nbr[p] == x && valid(g.eid[p]);
+    }
+    if (ok) return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+namespace ssq {
+
+void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kept, int threads) {
+  if (ne <= 0) return;
+  std::vector<int32_t> lab((size_t)(2 * ne));
+  for (int64_t i = 0; i < ne; i++) { lab[2 * i] = u[i]; lab[2 * i + 1] = v[i]; }
+  std::sort(lab.begin(), lab.end());
+  lab.erase(std::unique(lab.begin(), lab.end()), lab.end());
+  int64_t nv = (int64_t)lab.size();
+  std::vector<int32_t> eu((size_t)ne), ev((size_t)ne);
+  for (int64_t i = 0; i < ne; i++) {
+    eu[i] = (int32_t)(std::lower_bound(lab.begin(), lab.end(), u[i]) - lab.begin());
+    ev[i] = (int32_t)(std::lower_bound(lab.begin(), lab.end(), v[i]) - lab.begin());
+  }
+  Graph g;
+  g.off.assign((size_t)nv + 1, 0);
+  for (int64_t i = 0; i < ne; i++) { g.off[eu[i] + 1]++; g.off[ev[i] + 1]++; }
+  for (int64_t i = 0; i < nv; i++) g.off[i + 1] += g.off[i];
+  g.nbr.resize((size_t)(2 * ne));
+  g.eid.resize((size_t)(2 * ne));
+  {
+    std::vector<int64_t> fill(g.off.begin(), g.off.end() - 1);
+    for (int64_t i = 0; i < ne; i++) {
+      int64_t p = fill[eu[i]]++;
+      g.nbr[p] = ev[i]; g.eid[p] = (int32_t)i;
+      p = fill[ev[i]]++;
+      g.nbr[p] = eu[i]; g.eid[p] = (int32_t)i;
+    }
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int64_t a = 0; a < nv; a++) {
+      tmp.clear();
+      for (int64_t p = g.off[a]; p < g.off[a + 1]; p++) tmp.emplace_back(g.nbr[p], g.eid[p]);
+      std::sort(tmp.begin(), tmp.end());
+      for (int64_t p = g.off[a], k = 0; p < g.off[a + 1]; p++, k++) { g.nbr[p] = tmp[k].first; g.eid[p] = tmp[k].second; }
+    }
+  }
+  int64_t maxdeg = 1;
+  for (int64_t a = 0; a < nv; a++) maxdeg = std::max(maxdeg, g.off[a + 1] - g.off[a]);
+
+  // 1. main-loop decisions, in parallel
+  std::vector<uint8_t> main_crit((size_t)ne, 0);
+  main_crit[0] = 1;
+  if (threads < 1) threads = 1;
+  {
+    std::atomic<int64_t> next{1};
+    auto work = [&]() {
+      std::vector<int32_t> I((size_t)maxdeg);
+      for (;;) {
+        int64_t s = next.fetch_add(256);
+        if (s >= ne) break;
+        int64_t e = std::min(ne, s + 256);
+        for (int64_t i = s; i < e; i++) {
+          int32_t idx = (int32_t)i;
+          main_crit[i] = !dominated(g, eu[i], ev[i], [idx](int32_t x) { return x < idx; }, I.data());
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+  }
+
+  // 2. sequential cascades
+  g.crit.assign((size_t)ne, 0);
+  std::vector<int32_t> I((size_t)maxdeg), ea((size_t)maxdeg), eb((size_t)maxdeg);
+  std::vector<int32_t> stamp((size_t)ne, -1);
+  std::priority_queue<int32_t> heap;
+  auto three_clique = [&](int32_t thresh, int32_t a, int32_t b, int32_t sid) {
+    const uint8_t* crit = g.crit.data();
+    int64_t m = common(g, a, b, [thresh, crit](int32_t x) { return x < thresh || crit[x]; }, I.data(),
+                       ea.data(), eb.data());
+    for (int64_t k = 0; k < m; k++) {
+      int32_t e1 = ea[k], e2 = eb[k];
+      if (!crit[e1] && e1 < thresh && stamp[e1] != sid) { stamp[e1] = sid; heap.push(e1); }
+      if (!crit[e2] && e2 < thresh && stamp[e2] != sid) { stamp[e2] = sid; heap.push(e2); }
+    }
+  };
+  std::fill(kept, kept + ne, 0);
+  g.crit[0] = 1;
+  kept[0] = 1;
+  for (int32_t idx = 1; idx < (int32_t)ne; idx++) {
+    if (!main_crit[idx]) continue;
+    g.crit[idx] = 1;
+    kept[idx] = 1;
+    three_clique(idx, eu[idx], ev[idx], idx);
+    while (!heap.empty()) {
+      int32_t cur = heap.top();
+      heap.pop();
+      if (g.crit[cur]) continue;
+      const uint8_t* crit = g.crit.data();
+      if (dominated(g, eu[cur], ev[cur], [cur, crit](int32_t x) { return x < cur || crit[x]; }, I.data()))
+        continue;
+      g.crit[cur] = 1;
+      kept[cur] = 1;
+      three_clique(cur, eu[cur], ev[cur], idx);
+    }
+  }
+}
+
+}  // namespace ssq

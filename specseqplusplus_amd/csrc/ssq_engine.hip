@@ -1,0 +1,1529 @@
+// libssq engine: Z2 boundary-matrix reduction on MI355X (gfx950, wave64).
+//
+// Pipeline for one matrix (or a block-diagonal batch of matrices), all device-resident:
+//   model bit 1  edge collapse: edge list gathered on device, kept set from the host
+//                collapser, coface propagation on device            (specseq.h:410-438,
+//                                                                     kernel.h:190-198)
+//   model bit 0  high-dimension guided clearing of max-dim cells      (kernel.h:121-292)
+//   init passes  low / leftmost-one, apparent pairs, clearing, compression
+//                                                                     (kernel.h:444-520)
+//   page loop    per dimension from high to low, spectral-sequence pages of columns x
+//                rows; each page runs claim/add rounds until every active column is a
+//                pivot, zero or below the page's row floor            (specseq.h:592-704)
+//   twist        rows of the dimension's pivots are zeroed before the next dimension
+//                                                                     (specseq.h:1039-1046)
+//   pairs        (row, pivot column) in row order                     (specseq.h:1182-1193)
+//
+// Exactness of the concurrent rounds.  A claim round does atomicMin(piv[low[c]], c) for the
+// columns whose low is new; after it, every column whose low is owned by a smaller column
+// is in the add list.  During an add round nobody writes piv, and a column that is piv[l]
+// is never in the add list, so pivot columns are read-only while others add them.  Every
+// addition adds a column to the left (p < c), which keeps all lower-left ranks and hence the
+// pairs.  A chasing column keeps adding piv[low] while that is a smaller column.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ssq_internal.h"
+#include "ssq_kernels.h"
+
+namespace ssq {
+
+// ------------------------------------------------------------------------------------
+// device state
+// ------------------------------------------------------------------------------------
+constexpr int kAddWaves = 4;           // wavefronts per add workgroup
+constexpr int kTile = 1024;            // LDS entries per staging buffer
+constexpr int kTileBuf = kTile + kWave;
+constexpr int kMaxDims = 8;            // cell dimensions 0..7
+
+struct Dev {
+  int64_t n;
+  const int32_t* dims;
+  const int64_t* colptr;
+  int32_t* pool;       // column store: [input rows | arena]
+  int64_t* off;
+  int32_t* len;
+  int32_t* cap;
+  int32_t* low;
+  uint8_t* state;
+  uint8_t* removed;
+  uint8_t* flag;       // scratch per-cell flag (clear / negative)
+  uint32_t* piv;       // row -> column
+  uint32_t* leftone;   // row -> leftmost column containing it
+  int32_t* work[4];    // W0, W1, A0, A1
+  uint32_t* ctr;       // [0..1] cW, [2..3] cA, [4] apparent, [5] hd-cleared
+  unsigned long long* arena_used;  // ints used past arena_base
+  int64_t arena_base, arena_cap;
+  uint32_t* lvl_cnt;   // slots taken per level 1..3
+  int32_t lvl_cap[4];
+  uint32_t lvl_slots[4];
+  unsigned long long* stats;  // [0] additions, [1] algorithmic bytes
+  uint32_t* err;
+};
+
+#define GRID_STRIDE(i, n) \
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// ------------------------------------------------------------------------------------
+// allocation in the column store (level slots, then the exact-size overflow arena)
+// ------------------------------------------------------------------------------------
+__device__ bool store_alloc(const Dev& d, int32_t need, int64_t* off, int32_t* capo) {
+  for (int k = 1; k <= 3; k++) {
+    if (need <= d.lvl_cap[k]) {
+      const uint32_t s = atomicAdd(&d.lvl_cnt[k], 1u);
+      if (s < d.lvl_slots[k]) {
+        const unsigned long long o = atomicAdd(d.arena_used, (unsigned long long)d.lvl_cap[k]);
+        if ((int64_t)o + d.lvl_cap[k] <= d.arena_cap) {
+          *off = d.arena_base + (int64_t)o;
+          *capo = d.lvl_cap[k];
+          return true;
+        }
+        atomicOr(d.err, 1u);
+        return false;
+      }
+    }
+  }
+  const int64_t sz = ((int64_t)need + 63) & ~(int64_t)63;
+  const unsigned long long o = atomicAdd(d.arena_used, (unsigned long long)sz);
+  if ((int64_t)o + sz <= d.arena_cap) {
+    *off = d.arena_base + (int64_t)o;
+    *capo = (int32_t)sz;
+    return true;
+  }
+  atomicOr(d.err, 1u);
+  return false;
+}
+
+// wave-uniform allocation by lane 0
+__device__ bool wave_alloc(const Dev& d, int32_t need, int64_t* off, int32_t* capo) {
+  int64_t o = 0;
+  int32_t c = 0;
+  int ok = 0;
+  if (lane_id() == 0) ok = store_alloc(d, need, &o, &c) ? 1 : 0;
+  ok = wave_bcast(ok, 0);
+  o = wave_bcast64(o, 0);
+  c = wave_bcast(c, 0);
+  *off = o;
+  *capo = c;
+  return ok != 0;
+}
+
+// ------------------------------------------------------------------------------------
+// setup and init passes
+// ------------------------------------------------------------------------------------
+__global__ void k_setup(Dev d) {
+  GRID_STRIDE(c, d.n) {
+    const int64_t o = d.colptr[c];
+    const int32_t l = (int32_t)(d.colptr[c + 1] - o);
+    const bool rem = d.removed[c] != 0;
+    d.off[c] = o;
+    d.cap[c] = l;
+    d.len[c] = rem ? 0 : l;
+    d.low[c] = (rem || l == 0) ? -1 : d.pool[o + l - 1];
+    d.state[c] = (rem || l == 0) ? ST_ZERO : ST_PENDING;
+    d.piv[c] = kEmpty;
+    d.leftone[c] = kEmpty;
+    d.flag[c] = 0;
+    d.flag[d.n + c] = 0;
+  }
+}
+
+// Low[col] and leftmost-one per row (kernel.h:444-457)
+__global__ void k_leftone(Dev d) {
+  GRID_STRIDE(c, d.n) {
+    const int32_t l = d.len[c];
+    const int32_t* col = d.pool + d.off[c];
+    for (int32_t k = 0; k < l; k++) atomicMin(&d.leftone[col[k]], (uint32_t)c);
+  }
+}
+
+// Apparent pairs and the positive-facet flag (kernel.h:459-481): the youngest facet of a
+// column is positive (its column reduces to zero; flag[row] = 1); a column that is leftmost
+// in its low row is an apparent pair; a column leftmost in any row is negative
+// (flag[n + col] = 1).  The two flags live in separate bytes: plain stores of 1 only.
+__global__ void k_lookup(Dev d, const uint8_t* hdclear) {
+  GRID_STRIDE(c, d.n) {
+    const int32_t l = d.len[c];
+    if (l == 0) continue;
+    const int32_t lw = d.low[c];
+    d.flag[lw] = 1;
+    if (d.leftone[lw] == (uint32_t)c) {
+      d.piv[lw] = (uint32_t)c;
+      d.state[c] = ST_PIVOT;
+      atomicAdd(&d.ctr[4], 1u);
+    }
+    const int32_t* col = d.pool + d.off[c];
+    for (int32_t k = 0; k < l; k++)
+      if (d.leftone[col[k]] == (uint32_t)c) {
+        d.flag[d.n + c] = 1;
+        break;
+      }
+    if (hdclear && hdclear[c]) d.flag[c] = 1;
+  }
+}
+
+// Clearing (kernel.h:483-493): positive columns that are not pivots are zeroed.
+__global__ void k_clearing(Dev d) {
+  GRID_STRIDE(c, d.n) {
+    if (d.flag[c] && d.state[c] != ST_PIVOT) {
+      d.state[c] = ST_ZERO;
+      d.len[c] = 0;
+      d.low[c] = -1;
+    }
+  }
+}
+
+// Compression (kernel.h:495-520): rows of negative cells are removed from every column.
+__global__ void k_compress(Dev d) {
+  GRID_STRIDE(c, d.n) {
+    if (d.state[c] == ST_ZERO) continue;
+    int32_t* col = d.pool + d.off[c];
+    const int32_t l = d.len[c];
+    int32_t o = 0;
+    for (int32_t k = 0; k < l; k++) {
+      const int32_t r = col[k];
+      if (!d.flag[d.n + r]) col[o++] = r;
+    }
+    d.len[c] = o;
+    d.low[c] = o ? col[o - 1] : -1;
+    if (!o) d.state[c] = ST_ZERO;
+  }
+}
+
+__global__ void k_maxdim(int64_t n, const int32_t* dims, int32_t* out, uint32_t* bad) {
+  GRID_STRIDE(c, n) {
+    const int32_t x = dims[c];
+    if (x < 0 || x >= kMaxDims) atomicOr(bad, 1u);
+    else atomicMax(out, x);
+  }
+}
+
+// Per-dimension page bounds: max low over pending columns (+1) and min first row over
+// non-zero columns (specseq.h:365-409).
+__global__ void k_dimstats(Dev d, int32_t* maxrow, int32_t* minrow) {
+  GRID_STRIDE(c, d.n) {
+    const int32_t dm = d.dims[c];
+    if (dm < 0 || dm >= kMaxDims || d.len[c] == 0) continue;
+    if (d.state[c] == ST_PENDING) atomicMax(&maxrow[dm], d.low[c] + 1);
+    atomicMin(&minrow[dm], d.pool[d.off[c]]);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// ordered compaction by a small key (stable partition), used for the per-dimension
+// pending lists and for the pair list
+// ------------------------------------------------------------------------------------
+constexpr int kCompactBlock = 256;
+
+template <typename KeyF>
+__global__ void k_key_count(int64_t n, KeyF key, uint32_t* counts, int nkeys) {
+  __shared__ uint32_t s[kMaxDims];
+  if (threadIdx.x < (unsigned)nkeys) s[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
+  if (i < n) {
+    const int k = key(i);
+    if (k >= 0) atomicAdd(&s[k], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)nkeys) counts[(int64_t)blockIdx.x * nkeys + threadIdx.x] = s[threadIdx.x];
+}
+
+// exclusive scan over blocks, keys laid out key-major in the output: base of key k is the
+// total of all smaller keys.  One workgroup; totals[k] receives each key's count.
+__global__ void k_key_scan(int64_t nblocks, uint32_t* counts, int nkeys, int64_t* totals) {
+  __shared__ int64_t s_tot[kMaxDims];
+  __shared__ int64_t s_part[1024];
+  int64_t key_base = 0;
+  for (int k = 0; k < nkeys; k++) {
+    // per-thread sequential chunk, then block scan of chunk sums
+    const int64_t per = (nblocks + blockDim.x - 1) / blockDim.x;
+    const int64_t b0 = (int64_t)threadIdx.x * per;
+    const int64_t b1 = b0 + per < nblocks ? b0 + per : nblocks;
+    int64_t sum = 0;
+    for (int64_t b = b0; b < b1; b++) sum += counts[b * nkeys + k];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t run = 0;
+      for (unsigned t = 0; t < blockDim.x; t++) {
+        const int64_t v = s_part[t];
+        s_part[t] = run;
+        run += v;
+      }
+      s_tot[k] = run;
+    }
+    __syncthreads();
+    int64_t run = key_base + s_part[threadIdx.x];
+    for (int64_t b = b0; b < b1; b++) {
+      const uint32_t v = counts[b * nkeys + k];
+      counts[b * nkeys + k] = (uint32_t)run;  // base offsets fit: n < 2^31
+      run += v;
+    }
+    key_base += s_tot[k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nkeys; k++) totals[k] = s_tot[k];
+}
+
+template <typename KeyF, typename EmitF>
+__global__ void k_key_scatter(int64_t n, KeyF key, const uint32_t* bases, int nkeys, EmitF emit) {
+  __shared__ uint32_t s_wave[kCompactBlock / kWave][kMaxDims];
+  const int64_t i = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
+  const int k = i < n ? key(i) : -1;
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint32_t rank_in_wave = 0;
+  for (int kk = 0; kk < nkeys; kk++) {
+    const uint64_t m = __ballot(k == kk);
+    if (k == kk) rank_in_wave = (uint32_t)__popcll(m & lower);
+    if (lane == 0) s_wave[w][kk] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  if (k >= 0) {
+    uint32_t before = 0;
+    for (int ww = 0; ww < w; ww++) before += s_wave[ww][k];
+    emit(i, (int64_t)bases[(int64_t)blockIdx.x * nkeys + k] + before + rank_in_wave);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// page loop kernels
+// ------------------------------------------------------------------------------------
+// Active columns of the page: pending with low in the row band (specseq.h:583-607 scans
+// the same window every iteration; here the window is scanned once per page).
+__global__ void k_scan_window(Dev d, const int32_t* cols, int32_t ncols, int32_t minrow) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ncols; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    int32_t c = -1;
+    bool act = false;
+    if (i < ncols) {
+      c = cols[i];
+      act = d.state[c] == ST_PENDING && d.low[c] >= minrow;
+    }
+    wave_push(act, c, d.work[0], &d.ctr[0]);
+  }
+}
+
+// Claim round r: pivot claims by atomicMin on the pivot table (kernel.h:583-607).
+__global__ void k_claim(Dev d, int r) {
+  const int in = r & 1, nx = (r + 1) & 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    d.ctr[2 + nx] = 0;
+    d.ctr[nx] = 0;
+  }
+  const uint32_t cnt = d.ctr[in];
+  const int32_t* list = d.work[in];
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < cnt; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    bool add = false;
+    int32_t who = -1;
+    if (i < cnt) {
+      const int32_t c = list[i];
+      const uint32_t old = atomicMin(&d.piv[d.low[c]], (uint32_t)c);
+      if (old == kEmpty) {
+        d.state[c] = ST_PIVOT;
+      } else if (old > (uint32_t)c) {
+        d.state[c] = ST_PIVOT;
+        add = true;
+        who = (int32_t)old;  // displaced pivot must now add c
+      } else {
+        add = true;
+        who = c;
+      }
+    }
+    wave_push(add, who, d.work[2 + in], &d.ctr[2 + in]);
+  }
+}
+
+// Add round r: one wavefront per listed column; it adds piv[low] while that pivot is a
+// smaller column and its low stays in the row band (kernel.h:814-1065 does one addition per
+// warp, or a block-wide chase).  The running column stays in LDS between additions while it
+// and the pivot fit; longer ones merge tile by tile into a fresh store slot.
+__global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t minrow) {
+  __shared__ int32_t lds[kAddWaves][3][kTileBuf];
+  const int in = r & 1, nx = (r + 1) & 1;
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const uint32_t cnt = d.ctr[2 + in];
+  const int32_t* list = d.work[2 + in];
+  const int64_t gw = (int64_t)blockIdx.x * kAddWaves + w;
+  const int64_t nw = (int64_t)gridDim.x * kAddWaves;
+  int32_t* bufA = lds[w][0];
+  int32_t* bufB = lds[w][1];
+  int32_t* bufO = lds[w][2];
+  unsigned long long n_add = 0, n_bytes = 0;
+
+  for (int64_t it = gw; it < cnt; it += nw) {
+    const int32_t c = list[it];
+    int64_t a_off = d.off[c];
+    int32_t a_cap = d.cap[c];
+    int32_t a_len = d.len[c];
+    int32_t l = d.low[c];
+    bool in_lds = false;
+    bool failed = false;
+    int32_t* sA = bufA;
+    int32_t* sO = bufO;
+    for (;;) {
+      const uint32_t p = d.piv[l];
+      if (p == kEmpty || p >= (uint32_t)c) break;
+      const int32_t b_len = d.len[p];
+      const int32_t* B = d.pool + d.off[p];
+      int32_t r_len;
+      if (a_len + b_len <= kTile) {
+        if (!in_lds) {
+          wave_copy(sA, d.pool + a_off, a_len);
+          in_lds = true;
+        }
+        wave_copy(bufB, B, b_len);
+        wave_sync();
+        r_len = wave_xor_lds(sA, a_len, bufB, b_len, sO);
+        int32_t* t = sA;
+        sA = sO;
+        sO = t;
+      } else {
+        if (in_lds) {  // the running column goes back to the store first
+          if (a_len > a_cap && !wave_alloc(d, a_len, &a_off, &a_cap)) { failed = true; break; }
+          wave_copy(d.pool + a_off, sA, a_len);
+          in_lds = false;
+          __threadfence();
+        }
+        int64_t o_off;
+        int32_t o_cap;
+        if (!wave_alloc(d, a_len + b_len, &o_off, &o_cap)) { failed = true; break; }
+        // tiled merge path: chunks of A and B staged in LDS; kTile merged positions per tile
+        const int32_t* A = d.pool + a_off;
+        int32_t* O = d.pool + o_off;
+        int32_t ia = 0, jb = 0, written = 0;
+        while (ia < a_len || jb < b_len) {
+          const int32_t ca = min(kTile, a_len - ia), cb = min(kTile, b_len - jb);
+          wave_copy(bufA, A + ia, ca);
+          wave_copy(bufB, B + jb, cb);
+          wave_sync();
+          const int32_t K = min(kTile, ca + cb);
+          int32_t jE;
+          const int32_t iE = split_adj(bufA, ca, bufB, cb, K, &jE);
+          const int32_t m = wave_xor_lds(bufA, iE, bufB, jE, bufO);
+          wave_copy(O + written, bufO, m);
+          wave_sync();
+          written += m;
+          ia += iE;
+          jb += jE;
+        }
+        a_off = o_off;
+        a_cap = o_cap;
+        r_len = written;
+        __threadfence();
+      }
+      n_add += 1;
+      n_bytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
+      a_len = r_len;
+      if (a_len == 0) {
+        l = -1;
+        break;
+      }
+      l = in_lds ? sA[a_len - 1] : d.pool[a_off + a_len - 1];
+      if (l < minrow) break;
+    }
+    if (!failed && in_lds) {
+      if (a_len > a_cap && !wave_alloc(d, a_len, &a_off, &a_cap)) failed = true;
+      else wave_copy(d.pool + a_off, sA, a_len);
+    }
+    if (failed) {  // store exhausted: the call reports SSQ_ENOMEM; keep memory safe
+      if (lane == 0) {
+        d.len[c] = 0;
+        d.low[c] = -1;
+        d.state[c] = ST_ZERO;
+      }
+      continue;
+    }
+    bool again = false;
+    if (lane == 0) {
+      d.off[c] = a_off;
+      d.cap[c] = a_cap;
+      d.len[c] = a_len;
+      d.low[c] = a_len ? l : -1;
+      d.state[c] = a_len ? ST_PENDING : ST_ZERO;
+      if (a_len && l >= minrow) {
+        const uint32_t p = d.piv[l];
+        again = (p == kEmpty || p > (uint32_t)c);
+      }
+    }
+    if (lane == 0 && again) {
+      const uint32_t pos = atomicAdd(&d.ctr[nx], 1u);
+      d.work[nx][pos] = c;
+    }
+    wave_sync();
+  }
+  if (lane == 0 && n_add) {
+    atomicAdd(&d.stats[0], n_add);
+    atomicAdd(&d.stats[1], n_bytes);
+  }
+}
+
+// Twist clearing at the end of a dimension (specseq.h:1039-1046).
+__global__ void k_twist(Dev d, const int32_t* cols, int32_t ncols) {
+  GRID_STRIDE(i, ncols) {
+    const int32_t c = cols[i];
+    if (d.state[c] == ST_PIVOT) {
+      const int32_t l = d.low[c];
+      if (d.state[l] == ST_PENDING) d.state[l] = ST_ZERO;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// edge collapse support (model bit 1)
+// ------------------------------------------------------------------------------------
+__global__ void k_edge_ends(int64_t ne, const int32_t* edge_cols, const int64_t* colptr, const int32_t* rows,
+                            int32_t* u, int32_t* v, uint32_t* bad) {
+  GRID_STRIDE(i, ne) {
+    const int32_t c = edge_cols[i];
+    const int64_t o = colptr[c];
+    if (colptr[c + 1] - o != 2) {
+      atomicOr(bad, 1u);
+      u[i] = v[i] = 0;
+    } else {
+      u[i] = rows[o];
+      v[i] = rows[o + 1];
+    }
+  }
+}
+
+__global__ void k_mark_removed(int64_t ne, const int32_t* edge_cols, const uint8_t* kept, uint8_t* removed) {
+  GRID_STRIDE(i, ne) removed[edge_cols[i]] = kept[i] ? 0 : 1;
+}
+
+// Removal propagates to cofaces, one dimension at a time (kernel.h:190-198).
+__global__ void k_propagate(int64_t n, int32_t dim, const int32_t* dims, const int64_t* colptr,
+                            const int32_t* rows, uint8_t* removed) {
+  GRID_STRIDE(c, n) {
+    if (dims[c] != dim) continue;
+    for (int64_t k = colptr[c]; k < colptr[c + 1]; k++)
+      if (removed[rows[k]]) {
+        removed[c] = 1;
+        break;
+      }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// high-dimension guided clearing (model bit 0)
+// ------------------------------------------------------------------------------------
+constexpr int kMaxVerts = 8;  // vertices per cell handled by guided clearing (dim <= 6)
+
+// Vertex set of each cell of dimension dim from its first two facets (kernel.h:121-188),
+// stored sorted in vs[c*kMaxVerts ...].  Vertex ids are vertex ranks.
+__global__ void k_vertex_sets(int64_t n, int32_t dim, const int32_t* dims, const int64_t* colptr,
+                              const int32_t* rows, const int32_t* vrank, int32_t* vs, uint32_t* bad) {
+  GRID_STRIDE(c, n) {
+    if (dims[c] != dim) continue;
+    int32_t* out = vs + c * kMaxVerts;
+    if (dim == 0) {
+      out[0] = vrank[c];
+      continue;
+    }
+    const int64_t o = colptr[c];
+    if (colptr[c + 1] - o != dim + 1 || dims[rows[o]] != dim - 1 || dims[rows[o + 1]] != dim - 1) {
+      atomicOr(bad, 1u);
+      for (int k = 0; k <= dim; k++) out[k] = -1;
+      continue;
+    }
+    const int32_t* f0 = vs + (int64_t)rows[o] * kMaxVerts;
+    const int32_t* f1 = vs + (int64_t)rows[o + 1] * kMaxVerts;
+    // merge two sorted dim-vertex sets (they share dim-1 vertices)
+    int i = 0, j = 0, k = 0;
+    while ((i < dim || j < dim) && k <= dim) {
+      int32_t x = i < dim ? f0[i] : 0x7fffffff;
+      int32_t y = j < dim ? f1[j] : 0x7fffffff;
+      if (x == y) {
+        out[k++] = x;
+        i++;
+        j++;
+      } else if (x < y) {
+        out[k++] = x;
+        i++;
+      } else {
+        out[k++] = y;
+        j++;
+      }
+    }
+    if (k != dim + 1 || i != dim || j != dim) {
+      atomicOr(bad, 1u);
+      for (int t = 0; t <= dim; t++) out[t] = -1;
+    }
+  }
+}
+
+struct Key128 {
+  unsigned long long lo, hi;
+};
+
+__device__ __forceinline__ Key128 pack_key(const int32_t* v, int nv, int bits) {
+  Key128 k{0ull, 0ull};
+  for (int t = 0; t < nv; t++) {
+    const int sh = t * bits;
+    const unsigned long long x = (unsigned long long)(uint32_t)v[t];
+    if (sh < 64) {
+      k.lo |= x << sh;
+      if (sh + bits > 64) k.hi |= x >> (64 - sh);
+    } else {
+      k.hi |= x << (sh - 64);
+    }
+  }
+  return k;
+}
+
+__device__ __forceinline__ uint64_t hash_key(Key128 k) {
+  uint64_t h = k.lo * 0x9E3779B97F4A7C15ull ^ (k.hi + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full;
+  h ^= h >> 29;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 32;
+  return h;
+}
+
+// Hash of the max-dim cells that are still in the complex: key -> column.
+__global__ void k_hash_insert(int64_t n, int32_t dim, const int32_t* dims, const uint8_t* removed,
+                              const int32_t* vs, int bits, Key128* keys, int32_t* vals, uint64_t mask) {
+  GRID_STRIDE(c, n) {
+    if (dims[c] != dim || removed[c]) continue;
+    const int32_t* v = vs + c * kMaxVerts;
+    if (v[0] < 0) continue;
+    const Key128 key = pack_key(v, dim + 1, bits);
+    uint64_t h = hash_key(key) & mask;
+    for (;;) {
+      if (atomicCAS(&vals[h], -1, (int32_t)c) == -1) {
+        keys[h] = key;
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+__device__ __forceinline__ int32_t hash_find(const Key128* keys, const int32_t* vals, uint64_t mask, Key128 key) {
+  uint64_t h = hash_key(key) & mask;
+  for (;;) {
+    const int32_t v = vals[h];
+    if (v < 0) return -1;
+    if (keys[h].lo == key.lo && keys[h].hi == key.hi) return v;
+    h = (h + 1) & mask;
+  }
+}
+
+// Max-dim cell s is positive if some vertex x adjacent to all of its vertices gives a
+// (dim+1)-simplex whose other facets are all cells with index < s (kernel.h:203-292;
+// Lemma: the youngest facet of a boundary cycle is positive).  Neighbours of the vertex
+// with the fewest neighbours are walked from the highest id down, at most max_iter of them.
+__global__ void k_hd_clear(int64_t n, int32_t dim, const int32_t* dims, const uint8_t* removed,
+                           const int32_t* vs, int bits, const int64_t* adj_off, const int32_t* adj,
+                           const Key128* keys, const int32_t* vals, uint64_t mask, int32_t max_iter,
+                           uint8_t* hdclear, uint32_t* ncleared) {
+  GRID_STRIDE(c, n) {
+    if (dims[c] != dim || removed[c]) continue;
+    const int32_t* v = vs + c * kMaxVerts;
+    if (v[0] < 0) continue;
+    const int nv = dim + 1;
+    int best = 0;
+    for (int t = 1; t < nv; t++)
+      if (adj_off[v[t] + 1] - adj_off[v[t]] < adj_off[v[best] + 1] - adj_off[v[best]]) best = t;
+    const int64_t b0 = adj_off[v[best]];
+    int64_t p = adj_off[v[best] + 1] - 1;
+    bool cleared = false;
+    for (int32_t it = 0; it < max_iter && p >= b0 && !cleared; it++, p--) {
+      const int32_t x = adj[p];
+      bool ok = true;
+      for (int t = 0; t < nv && ok; t++) {
+        if (t == best) continue;
+        if (v[t] == x) { ok = false; break; }
+        // binary search x in adj list of v[t]
+        int64_t lo = adj_off[v[t]], hi = adj_off[v[t] + 1];
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (adj[mid] < x) lo = mid + 1;
+          else hi = mid;
+        }
+        ok = lo < adj_off[v[t] + 1] && adj[lo] == x;
+      }
+      if (!ok) continue;
+      // tau = v + {x}; every facet other than s itself must exist with index < c
+      int32_t tau[kMaxVerts + 1];
+      int k = 0;
+      bool put = false;
+      for (int t = 0; t < nv; t++) {
+        if (!put && x < v[t]) {
+          tau[k++] = x;
+          put = true;
+        }
+        tau[k++] = v[t];
+      }
+      if (!put) tau[k++] = x;
+      bool all = true;
+      for (int drop = 0; drop <= nv && all; drop++) {
+        if (tau[drop] == x) continue;  // that facet is s itself
+        int32_t f[kMaxVerts];
+        int q = 0;
+        for (int t = 0; t <= nv; t++)
+          if (t != drop) f[q++] = tau[t];
+        const int32_t col = hash_find(keys, vals, mask, pack_key(f, nv, bits));
+        all = col >= 0 && col < c;
+      }
+      if (all) cleared = true;
+    }
+    if (cleared) {
+      hdclear[c] = 1;
+      atomicAdd(ncleared, 1u);
+    }
+  }
+}
+
+// adjacency (CSR over vertex ranks) of the edges still in the complex
+__global__ void k_adj_count(int64_t ne, const int32_t* eu, const int32_t* ev, const int32_t* edge_cols,
+                            const uint8_t* removed, int64_t* deg) {
+  GRID_STRIDE(i, ne) {
+    if (removed[edge_cols[i]]) continue;
+    atomicAdd((unsigned long long*)&deg[eu[i]], 1ull);
+    atomicAdd((unsigned long long*)&deg[ev[i]], 1ull);
+  }
+}
+__global__ void k_adj_fill(int64_t ne, const int32_t* eu, const int32_t* ev, const int32_t* edge_cols,
+                           const uint8_t* removed, const int64_t* off, unsigned long long* fill, int32_t* adj) {
+  GRID_STRIDE(i, ne) {
+    if (removed[edge_cols[i]]) continue;
+    const int32_t a = eu[i], b = ev[i];
+    adj[off[a] + (int64_t)atomicAdd(&fill[a], 1ull)] = b;
+    adj[off[b] + (int64_t)atomicAdd(&fill[b], 1ull)] = a;
+  }
+}
+__global__ void k_adj_sort(int64_t nv, const int64_t* off, int32_t* adj) {
+  GRID_STRIDE(a, nv) {
+    const int64_t b = off[a], e = off[a + 1];
+    for (int64_t i = b + 1; i < e; i++) {
+      const int32_t x = adj[i];
+      int64_t j = i - 1;
+      while (j >= b && adj[j] > x) {
+        adj[j + 1] = adj[j];
+        j--;
+      }
+      adj[j + 1] = x;
+    }
+  }
+}
+__global__ void k_vrank_ends(int64_t ne, const int32_t* vrank, int32_t* eu, int32_t* ev, uint32_t* bad) {
+  GRID_STRIDE(i, ne) {
+    const int32_t a = vrank[eu[i]], b = vrank[ev[i]];
+    if (a < 0 || b < 0) atomicOr(bad, 1u);  // an edge endpoint that is not a vertex
+    eu[i] = a < 0 ? 0 : a;
+    ev[i] = b < 0 ? 0 : b;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));                   \
+      return SSQ_EDEVICE;                                                          \
+    }                                                                              \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device) : device_(device) {}
+  ~Engine() { release(); }
+
+  int init() {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+      err_ = "no HIP device";
+      return SSQ_EDEVICE;
+    }
+    if (device_ < 0 || device_ >= ndev) {
+      err_ = "bad device index";
+      return SSQ_EINVAL;
+    }
+    HIPCHK(hipSetDevice(device_));
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc(&host_ctr_, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device_));
+    cus_ = prop.multiProcessorCount;
+    return SSQ_OK;
+  }
+
+  const std::string& error() const { return err_; }
+  hipStream_t stream() const { return stream_; }
+
+  int reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
+                    const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
+                    ssq_stats* st, int64_t* d_essential = nullptr, int64_t* ne_out = nullptr);
+
+ private:
+  void set_error(const std::string& s) { err_ = s; }
+  template <typename T>
+  int ensure(DevBuf& b, size_t count, T** out) {
+    const size_t need = std::max<size_t>(count, 1) * sizeof(T);
+    if (b.bytes < need) {
+      if (b.p) (void)hipFree(b.p);
+      b.p = nullptr;
+      b.bytes = 0;
+      const size_t alloc = need + need / 8;
+      if (hipMalloc(&b.p, alloc) != hipSuccess) {
+        (void)hipGetLastError();
+        err_ = "device allocation of " + std::to_string(alloc) + " bytes failed";
+        return SSQ_ENOMEM;
+      }
+      b.bytes = alloc;
+    }
+    *out = (T*)b.p;
+    return SSQ_OK;
+  }
+  void release() {
+    for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
+                      &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
+                      &b_hash_, &b_hd_})
+      if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
+    if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
+    if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
+  }
+  int sync_read(const uint32_t* dsrc, int count) {
+    HIPCHK(hipMemcpyAsync(host_ctr_, dsrc, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return SSQ_OK;
+  }
+  int grid_for(int64_t n, int block = 256) const {
+    int64_t g = (n + block - 1) / block;
+    const int64_t cap = (int64_t)cus_ * 16;
+    return (int)std::max<int64_t>(1, std::min(g, cap));
+  }
+  template <typename KeyF, typename EmitF>
+  int compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host);
+  int collapse(Dev& d, int64_t n, const int32_t* d_dims, const int64_t* d_colptr, const int32_t* d_rows,
+               ssq_stats* st);
+  int guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dims, const int64_t* d_colptr,
+                   const int32_t* d_rows, const ssq_params& p, uint8_t** hd_out, ssq_stats* st);
+
+  int device_ = 0;
+  int cus_ = 256;
+  hipStream_t stream_ = nullptr;
+  uint32_t* host_ctr_ = nullptr;
+  std::string err_;
+  DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
+      b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_;
+  int64_t arena_hint_ = 0;
+};
+
+template <typename KeyF, typename EmitF>
+int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host) {
+  const int64_t nb = std::max<int64_t>(1, (n + kCompactBlock - 1) / kCompactBlock);
+  uint32_t* counts;
+  int rc = ensure(b_cnt_, (size_t)(nb * nkeys + 2 * kMaxDims + 16), &counts);
+  if (rc) return rc;
+  int64_t* totals = (int64_t*)(counts + nb * nkeys + (nb * nkeys) % 2);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_key_count<KeyF>, dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key, counts,
+                       nkeys);
+  } else {
+    HIPCHK(hipMemsetAsync(counts, 0, sizeof(uint32_t) * nkeys, stream_));
+  }
+  hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(1024), 0, stream_, nb, counts, nkeys, totals);
+  if (n > 0)
+    hipLaunchKernelGGL((k_key_scatter<KeyF, EmitF>), dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key,
+                       counts, nkeys, emit);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host_ctr_, totals, sizeof(int64_t) * nkeys, hipMemcpyDeviceToHost, stream_));
+  HIPCHK(hipStreamSynchronize(stream_));
+  std::memcpy(totals_host, host_ctr_, sizeof(int64_t) * nkeys);
+  return SSQ_OK;
+}
+
+struct KeyPendingDim {
+  const uint8_t* state;
+  const int32_t* dims;
+  __device__ int operator()(int64_t c) const {
+    const int32_t d = dims[c];
+    return (state[c] == ST_PENDING && d >= 1 && d < kMaxDims) ? d : -1;
+  }
+};
+struct EmitIndex {
+  int32_t* out;
+  __device__ void operator()(int64_t i, int64_t pos) const { out[pos] = (int32_t)i; }
+};
+struct KeyDimEq {
+  const int32_t* dims;
+  int32_t dim;
+  __device__ int operator()(int64_t c) const { return dims[c] == dim ? 0 : -1; }
+};
+struct KeyPivot {
+  const uint32_t* piv;
+  __device__ int operator()(int64_t r) const { return piv[r] != kEmpty ? 0 : -1; }
+};
+struct EmitPair {
+  const uint32_t* piv;
+  int64_t* out;
+  __device__ void operator()(int64_t r, int64_t pos) const {
+    out[2 * pos] = r;
+    out[2 * pos + 1] = (int64_t)piv[r];
+  }
+};
+struct KeyEssential {
+  const uint32_t* piv;
+  const uint8_t* removed;
+  const uint8_t* state;
+  const int32_t* low;
+  __device__ int operator()(int64_t c) const {
+    if (removed[c] || piv[c] != kEmpty) return -1;
+    if (state[c] == ST_PIVOT && low[c] >= 0 && piv[low[c]] == (uint32_t)c) return -1;
+    return 0;
+  }
+};
+struct EmitI64 {
+  int64_t* out;
+  __device__ void operator()(int64_t i, int64_t pos) const { out[pos] = i; }
+};
+struct KeyVertex {
+  const int32_t* dims;
+  __device__ int operator()(int64_t c) const { return dims[c] == 0 ? 0 : -1; }
+};
+struct EmitRank {
+  int32_t* vrank;
+  __device__ void operator()(int64_t i, int64_t pos) const { vrank[i] = (int32_t)pos; }
+};
+
+int Engine::collapse(Dev& d, int64_t n, const int32_t* d_dims, const int64_t* d_colptr, const int32_t* d_rows,
+                     ssq_stats* st) {
+  int32_t* ecols;
+  int rc = ensure(b_e_, (size_t)(3 * n + 16), &ecols);
+  if (rc) return rc;
+  int64_t ne = 0;
+  rc = compact(n, 1, KeyDimEq{d_dims, 1}, EmitIndex{ecols}, &ne);
+  if (rc) return rc;
+  if (ne == 0) return SSQ_OK;
+  int32_t* eu = ecols + n;
+  int32_t* ev = ecols + 2 * n;
+  HIPCHK(hipMemsetAsync(d.ctr + 8, 0, sizeof(uint32_t), stream_));
+  hipLaunchKernelGGL(k_edge_ends, dim3(grid_for(ne)), dim3(256), 0, stream_, ne, ecols, d_colptr, d_rows, eu, ev,
+                     d.ctr + 8);
+  std::vector<int32_t> hu((size_t)ne), hv((size_t)ne);
+  std::vector<uint8_t> kept((size_t)ne);
+  HIPCHK(hipMemcpyAsync(hu.data(), eu, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, stream_));
+  HIPCHK(hipMemcpyAsync(hv.data(), ev, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, stream_));
+  rc = sync_read(d.ctr + 8, 1);
+  if (rc) return rc;
+  if (host_ctr_[0]) {
+    err_ = "edge column without exactly two boundary vertices";
+    return SSQ_EFORMAT;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  collapse_edges(ne, hu.data(), hv.data(), kept.data(), threads);
+  auto t1 = std::chrono::steady_clock::now();
+  int64_t nk = 0;
+  for (uint8_t k : kept) nk += k;
+  if (st) {
+    st->t_collapse_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    st->n_kept_edges += nk;
+  }
+  HIPCHK(hipMemcpyAsync(d.flag, kept.data(), (size_t)ne, hipMemcpyHostToDevice, stream_));
+  hipLaunchKernelGGL(k_mark_removed, dim3(grid_for(ne)), dim3(256), 0, stream_, ne, ecols, d.flag, d.removed);
+  int32_t maxdim = 0;
+  for (int dd = 2; dd < kMaxDims; dd++) {
+    hipLaunchKernelGGL(k_propagate, dim3(grid_for(n)), dim3(256), 0, stream_, n, dd, d_dims, d_colptr, d_rows,
+                       d.removed);
+    maxdim = dd;
+  }
+  (void)maxdim;
+  HIPCHK(hipStreamSynchronize(stream_));  // kept[] is a host temporary
+  HIPCHK(hipGetLastError());
+  return SSQ_OK;
+}
+
+int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dims, const int64_t* d_colptr,
+                         const int32_t* d_rows, const ssq_params& p, uint8_t** hd_out, ssq_stats* st) {
+  *hd_out = nullptr;
+  if (maxdim < 1 || maxdim + 1 > kMaxVerts) return SSQ_OK;
+  uint8_t* hd;
+  int rc = ensure(b_hd_, (size_t)n, &hd);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(hd, 0, (size_t)n, stream_));
+  // vertex ranks
+  int32_t* vs;
+  rc = ensure(b_vs_, (size_t)(n * (kMaxVerts + 1) + 16), &vs);
+  if (rc) return rc;
+  int32_t* vrank = vs + n * kMaxVerts;
+  HIPCHK(hipMemsetAsync(vrank, 0xff, sizeof(int32_t) * n, stream_));
+  int64_t nv = 0;
+  rc = compact(n, 1, KeyVertex{d_dims}, EmitRank{vrank}, &nv);
+  if (rc) return rc;
+  int bits = 1;
+  while (bits < 31 && (1ll << bits) < std::max<int64_t>(nv, 2)) bits++;
+  if ((int64_t)bits * (maxdim + 1) > 128) return SSQ_OK;  // keys would not fit 128 bits
+  HIPCHK(hipMemsetAsync(d.ctr + 8, 0, sizeof(uint32_t) * 2, stream_));
+  for (int32_t dd = 0; dd <= maxdim; dd++)
+    hipLaunchKernelGGL(k_vertex_sets, dim3(grid_for(n)), dim3(256), 0, stream_, n, dd, d_dims, d_colptr, d_rows,
+                       vrank, vs, d.ctr + 8);
+  // edges (in column order) -> adjacency over vertex ranks, kept edges only
+  int32_t* ecols;
+  rc = ensure(b_e_, (size_t)(3 * n + 16), &ecols);
+  if (rc) return rc;
+  int64_t ne = 0;
+  rc = compact(n, 1, KeyDimEq{d_dims, 1}, EmitIndex{ecols}, &ne);
+  if (rc) return rc;
+  int32_t* eu = ecols + n;
+  int32_t* ev = ecols + 2 * n;
+  hipLaunchKernelGGL(k_edge_ends, dim3(grid_for(std::max<int64_t>(ne, 1))), dim3(256), 0, stream_, ne, ecols,
+                     d_colptr, d_rows, eu, ev, d.ctr + 9);
+  hipLaunchKernelGGL(k_vrank_ends, dim3(grid_for(std::max<int64_t>(ne, 1))), dim3(256), 0, stream_, ne, vrank, eu,
+                     ev, d.ctr + 9);
+  rc = sync_read(d.ctr + 8, 2);
+  if (rc) return rc;
+  if (host_ctr_[0] || host_ctr_[1]) return SSQ_OK;  // not a simplicial complex: no guided clearing
+  int64_t* aoff;
+  rc = ensure(b_adj_, (size_t)(2 * (nv + 1) + ne + 8), &aoff);
+  if (rc) return rc;
+  unsigned long long* fill = (unsigned long long*)(aoff + nv + 1);
+  int32_t* adj = (int32_t*)(aoff + 2 * (nv + 1));
+  HIPCHK(hipMemsetAsync(aoff, 0, sizeof(int64_t) * 2 * (nv + 1), stream_));
+  hipLaunchKernelGGL(k_adj_count, dim3(grid_for(std::max<int64_t>(ne, 1))), dim3(256), 0, stream_, ne, eu, ev, ecols,
+                     d.removed, aoff + 1);
+  // exclusive scan of degrees on the host (nv is the vertex count; small)
+  std::vector<int64_t> deg((size_t)nv + 1);
+  HIPCHK(hipMemcpyAsync(deg.data(), aoff, sizeof(int64_t) * (nv + 1), hipMemcpyDeviceToHost, stream_));
+  HIPCHK(hipStreamSynchronize(stream_));
+  for (int64_t a = 0; a < nv; a++) deg[a + 1] += deg[a];
+  HIPCHK(hipMemcpyAsync(aoff, deg.data(), sizeof(int64_t) * (nv + 1), hipMemcpyHostToDevice, stream_));
+  hipLaunchKernelGGL(k_adj_fill, dim3(grid_for(std::max<int64_t>(ne, 1))), dim3(256), 0, stream_, ne, eu, ev, ecols,
+                     d.removed, aoff, fill, adj);
+  hipLaunchKernelGGL(k_adj_sort, dim3(grid_for(nv)), dim3(256), 0, stream_, nv, aoff, adj);
+  // hash of max-dim cells
+  int64_t nmax = 0;
+  {
+    int64_t cnt[1];
+    rc = compact(n, 1, KeyDimEq{d_dims, maxdim}, EmitIndex{ecols}, cnt);  // ecols reused as scratch
+    if (rc) return rc;
+    nmax = cnt[0];
+  }
+  uint64_t hsize = 1;
+  while (hsize < (uint64_t)(2 * nmax + 2)) hsize <<= 1;
+  Key128* keys;
+  rc = ensure(b_hash_, (size_t)(hsize * (sizeof(Key128) + sizeof(int32_t))), (uint8_t**)&keys);
+  if (rc) return rc;
+  int32_t* vals = (int32_t*)(keys + hsize);
+  HIPCHK(hipMemsetAsync(vals, 0xff, sizeof(int32_t) * hsize, stream_));
+  hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(n)), dim3(256), 0, stream_, n, maxdim, d_dims, d.removed, vs, bits,
+                     keys, vals, (uint64_t)(hsize - 1));
+  HIPCHK(hipMemsetAsync(d.ctr + 5, 0, sizeof(uint32_t), stream_));
+  hipLaunchKernelGGL(k_hd_clear, dim3(grid_for(n)), dim3(256), 0, stream_, n, maxdim, d_dims, d.removed, vs, bits,
+                     aoff, adj, keys, vals, (uint64_t)(hsize - 1), p.max_iteration, hd, d.ctr + 5);
+  HIPCHK(hipGetLastError());
+  rc = sync_read(d.ctr + 5, 1);
+  if (rc) return rc;
+  if (st) st->n_cleared_hd += host_ctr_[0];
+  *hd_out = hd;
+  return SSQ_OK;
+}
+
+int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
+                          const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
+                          ssq_stats* st, int64_t* d_essential, int64_t* ne_out) {
+  err_.clear();
+  if (n < 0 || nnz < 0 || n >= 0x7fffffffLL || !np_out || (n > 0 && (!d_colptr || !d_dims || !d_pairs))) {
+    err_ = "invalid arguments";
+    return SSQ_EINVAL;
+  }
+  if (p.ss_block_size < 1 || !(p.growth >= 1.0f) || p.queue_size < 1 || p.level1 < 1 || p.level2 < 1 ||
+      p.max_iteration < 0 || (p.model & ~3) || (p.dualize & ~1)) {
+    err_ = "invalid parameters";
+    return SSQ_EINVAL;
+  }
+  HIPCHK(hipSetDevice(device_));
+  auto T0 = std::chrono::steady_clock::now();
+  *np_out = 0;
+  if (ne_out) *ne_out = 0;
+  if (n == 0) return SSQ_OK;
+
+  Dev d{};
+  d.n = n;
+  d.dims = d_dims;
+  d.colptr = d_colptr;
+  int rc;
+  if ((rc = ensure(b_off_, (size_t)n, &d.off)) || (rc = ensure(b_len_, (size_t)n, &d.len)) ||
+      (rc = ensure(b_cap_, (size_t)n, &d.cap)) || (rc = ensure(b_low_, (size_t)n, &d.low)) ||
+      (rc = ensure(b_state_, (size_t)n, &d.state)) || (rc = ensure(b_removed_, (size_t)n, &d.removed)) ||
+      (rc = ensure(b_flag_, (size_t)(2 * n), &d.flag)) || (rc = ensure(b_piv_, (size_t)n, &d.piv)) ||
+      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)64, &d.ctr)))
+    return rc;
+  int32_t* U;
+  if ((rc = ensure(b_U_, (size_t)n, &U))) return rc;
+  HIPCHK(hipMemsetAsync(d.ctr, 0, 64 * sizeof(uint32_t), stream_));
+  d.stats = (unsigned long long*)(d.ctr + 32);
+  d.arena_used = (unsigned long long*)(d.ctr + 36);
+  d.lvl_cnt = d.ctr + 40;
+  d.err = d.ctr + 44;
+  HIPCHK(hipMemsetAsync(d.removed, 0, (size_t)n, stream_));
+
+  // max dimension (and range check of the cell dimensions)
+  int32_t maxdim = 0;
+  {
+    hipLaunchKernelGGL(k_maxdim, dim3(grid_for(n)), dim3(256), 0, stream_, n, d_dims, (int32_t*)(d.ctr + 48),
+                       d.ctr + 49);
+    HIPCHK(hipGetLastError());
+    rc = sync_read(d.ctr + 48, 2);
+    if (rc) return rc;
+    if (host_ctr_[1]) {
+      err_ = "cell dimension outside [0, 7]";
+      return SSQ_EFORMAT;
+    }
+    maxdim = (int32_t)host_ctr_[0];
+  }
+
+  // column store: input rows first, then the arena
+  const int64_t arena0 = std::max<int64_t>({arena_hint_, 4 * nnz + 4096, (int64_t)1 << 22});
+  int64_t arena_cap = arena0;
+
+  ssq_stats local{};
+  ssq_stats* S = st ? st : &local;
+  std::memset(S, 0, sizeof(*S));
+
+  if ((p.model & 2) && maxdim >= 1) {
+    rc = collapse(d, n, d_dims, d_colptr, d_rows, S);
+    if (rc) return rc;
+  }
+  uint8_t* hd = nullptr;
+  if ((p.model & 1) && maxdim >= 1) {
+    rc = guided_clear(d, n, maxdim, d_dims, d_colptr, d_rows, p, &hd, S);
+    if (rc) return rc;
+  }
+
+  for (int attempt = 0;; attempt++) {
+    if ((rc = ensure(b_pool_, (size_t)(nnz + arena_cap), &d.pool))) return rc;
+    d.arena_base = nnz;
+    d.arena_cap = arena_cap;
+    HIPCHK(hipMemcpyAsync(d.pool, d_rows, sizeof(int32_t) * std::max<int64_t>(nnz, 0), hipMemcpyDeviceToDevice,
+                          stream_));
+    HIPCHK(hipMemsetAsync(d.ctr, 0, 64 * sizeof(uint32_t), stream_));
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d);
+    hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
+    hipLaunchKernelGGL(k_lookup, dim3(g), dim3(256), 0, stream_, d, (const uint8_t*)hd);
+    hipLaunchKernelGGL(k_clearing, dim3(g), dim3(256), 0, stream_, d);
+    hipLaunchKernelGGL(k_compress, dim3(g), dim3(256), 0, stream_, d);
+    HIPCHK(hipGetLastError());
+    int64_t cnt[kMaxDims] = {0};
+    rc = compact(n, kMaxDims, KeyPendingDim{d.state, d.dims}, EmitIndex{U}, cnt);
+    if (rc) return rc;
+    int64_t ubase[kMaxDims];
+    int64_t upend = 0;
+    for (int k = 0; k < kMaxDims; k++) {
+      ubase[k] = upend;
+      upend += cnt[k];
+    }
+    // page bounds per dimension
+    int32_t* rowstat = (int32_t*)(d.ctr + 16);
+    {
+      std::vector<int32_t> init(2 * kMaxDims);
+      for (int k = 0; k < kMaxDims; k++) {
+        init[k] = -1;
+        init[kMaxDims + k] = 0x7fffffff;
+      }
+      HIPCHK(hipMemcpyAsync(rowstat, init.data(), sizeof(int32_t) * 2 * kMaxDims, hipMemcpyHostToDevice, stream_));
+      hipLaunchKernelGGL(k_dimstats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
+      rc = sync_read(d.ctr + 4, 16 + 2 * kMaxDims - 4);
+      if (rc) return rc;
+    }
+    int32_t maxrow[kMaxDims], minrow[kMaxDims];
+    for (int k = 0; k < kMaxDims; k++) {
+      maxrow[k] = (int32_t)host_ctr_[12 + k];
+      minrow[k] = (int32_t)host_ctr_[12 + kMaxDims + k];
+    }
+    S->n_apparent = host_ctr_[0];
+    S->n_pending = upend;
+    // level slots (specseq.h:281-283), arena-backed
+    d.lvl_cap[1] = p.level1;
+    d.lvl_cap[2] = p.level2;
+    d.lvl_cap[3] = (int32_t)std::min<int64_t>((int64_t)p.level2 * 8, 0x3fffffff);
+    const int64_t U1 = upend;
+    d.lvl_slots[1] = (uint32_t)std::min<int64_t>(U1 + 1, 0xffffffffLL);
+    d.lvl_slots[2] = (uint32_t)std::min<int64_t>(U1 / std::max(1, p.level2 / p.level1) / 2 + 1, 0xffffffffLL);
+    d.lvl_slots[3] = (uint32_t)std::min<int64_t>(U1 / std::max<int64_t>(1, (int64_t)d.lvl_cap[3] / p.level1) / 2 + 1,
+                                                 0xffffffffLL);
+    auto T1 = std::chrono::steady_clock::now();
+    S->t_init_ms = std::chrono::duration<double, std::milli>(T1 - T0).count();
+
+    // page loop, dimensions high to low (specseq.h:598-704)
+    int32_t* W0;
+    if ((rc = ensure(b_work_, (size_t)(4 * std::max<int64_t>(upend, 1) + 64), &W0))) return rc;
+    const int64_t wcap = std::max<int64_t>(upend, 1);
+    for (int k = 0; k < 4; k++) d.work[k] = W0 + k * wcap;
+    const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * 12);
+    const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
+    const int rounds_per_sync = 2;
+    bool oom = false;
+    for (int32_t dim = maxdim; dim >= 1 && !oom; dim--) {
+      const int64_t L = cnt[dim];
+      if (L == 0 || maxrow[dim] < 0) continue;
+      const int32_t* cols = U + ubase[dim];
+      int32_t ss = p.ss_block_size;
+      int64_t begin = 0, end = std::min<int64_t>(ss, L);
+      int64_t mrow = (int64_t)maxrow[dim] - ss, xrow = mrow + ss;
+      const int64_t dminrow = minrow[dim];
+      while ((begin < L || xrow > dminrow) && !oom) {
+        HIPCHK(hipMemsetAsync(d.ctr, 0, 4 * sizeof(uint32_t), stream_));
+        const int64_t wn = end - begin;
+        if (wn > 0) {
+          const int32_t fl = (int32_t)std::max<int64_t>(mrow, INT32_MIN);
+          hipLaunchKernelGGL(k_scan_window, dim3(grid_for(wn)), dim3(256), 0, stream_, d, cols + begin, (int32_t)wn,
+                             fl);
+          const int cg = grid_for(wn);
+          int r = 0;
+          for (;;) {
+            for (int t = 0; t < rounds_per_sync; t++, r++) {
+              hipLaunchKernelGGL(k_claim, dim3(cg), dim3(256), 0, stream_, d, r);
+              hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(kWave * kAddWaves), 0, stream_, d, r, fl);
+            }
+            HIPCHK(hipGetLastError());
+            rc = sync_read(d.ctr, 64);
+            if (rc) return rc;
+            S->n_rounds += rounds_per_sync;
+            if (host_ctr_[44]) {
+              oom = true;
+              break;
+            }
+            if (host_ctr_[r & 1] == 0) break;
+            if (r > (1 << 24)) {
+              err_ = "page did not converge";
+              return SSQ_EDEVICE;
+            }
+          }
+        }
+        S->n_pages++;
+        ss = (int32_t)((float)ss * p.growth);
+        if (ss < 1) ss = 1;
+        if (xrow <= dminrow) begin = end;
+        else {
+          mrow -= ss;
+          xrow -= ss;
+        }
+        if (end < L) end = std::min<int64_t>(end + ss, L);
+      }
+      if (oom) break;
+      hipLaunchKernelGGL(k_twist, dim3(grid_for(L)), dim3(256), 0, stream_, d, cols, (int32_t)L);
+    }
+    if (oom) {
+      const int64_t budget = (int64_t)1 << 36;  // 256 GiB of int32 entries at most
+      if (arena_cap * 2 > budget || attempt > 8) {
+        err_ = "column store exhausted";
+        return SSQ_ENOMEM;
+      }
+      arena_cap *= 2;
+      S->n_pages = S->n_rounds = 0;
+      continue;  // rerun from the matrix with a larger arena
+    }
+    // pairs in row order
+    int64_t np = 0;
+    rc = compact(n, 1, KeyPivot{d.piv}, EmitPair{d.piv, d_pairs}, &np);
+    if (rc) return rc;
+    *np_out = np;
+    if (d_essential && ne_out) {
+      int64_t ne = 0;
+      rc = compact(n, 1, KeyEssential{d.piv, d.removed, d.state, d.low}, EmitI64{d_essential}, &ne);
+      if (rc) return rc;
+      *ne_out = ne;
+    }
+    rc = sync_read(d.ctr, 64);
+    if (rc) return rc;
+    const unsigned long long* s64 = (const unsigned long long*)(host_ctr_ + 32);
+    S->n_additions = (int64_t)s64[0];
+    S->add_bytes = (double)s64[1];
+    S->store_bytes = (int64_t)(nnz + arena_cap) * 4;
+    arena_hint_ = std::max(arena_hint_, (int64_t)(*(const unsigned long long*)(host_ctr_ + 36)) * 2);
+    auto T2 = std::chrono::steady_clock::now();
+    S->t_reduce_ms = std::chrono::duration<double, std::milli>(T2 - T1).count();
+    S->t_total_ms = std::chrono::duration<double, std::milli>(T2 - T0).count();
+    return SSQ_OK;
+  }
+}
+
+// ====================================================================================
+// C ABI
+// ====================================================================================
+struct Handle {
+  Engine eng;
+  std::string err;
+  DevBuf in_colptr, in_rows, in_dims, out_pairs, out_ess;
+  explicit Handle(int dev) : eng(dev) {}
+};
+
+static int upload(Handle* H, const int64_t* colptr, const int32_t* rows, const int32_t* dims, int64_t n,
+                  int64_t nnz, int64_t** dc, int32_t** dr, int32_t** dd, int64_t** dp, int64_t** de) {
+  auto grow = [&](DevBuf& b, size_t bytes) -> int {
+    bytes = std::max<size_t>(bytes, 8);
+    if (b.bytes >= bytes) return SSQ_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.p, bytes + bytes / 8) != hipSuccess) {
+      (void)hipGetLastError();
+      H->err = "device allocation failed";
+      return SSQ_ENOMEM;
+    }
+    b.bytes = bytes + bytes / 8;
+    return SSQ_OK;
+  };
+  int rc;
+  if ((rc = grow(H->in_colptr, sizeof(int64_t) * (n + 1))) || (rc = grow(H->in_rows, sizeof(int32_t) * nnz)) ||
+      (rc = grow(H->in_dims, sizeof(int32_t) * n)) || (rc = grow(H->out_pairs, sizeof(int64_t) * 2 * n)) ||
+      (rc = grow(H->out_ess, sizeof(int64_t) * n)))
+    return rc;
+  hipStream_t s = H->eng.stream();
+  if (hipMemcpyAsync(H->in_colptr.p, colptr, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+      (nnz && hipMemcpyAsync(H->in_rows.p, rows, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s) != hipSuccess) ||
+      (n && hipMemcpyAsync(H->in_dims.p, dims, sizeof(int32_t) * n, hipMemcpyHostToDevice, s) != hipSuccess)) {
+    H->err = "host to device copy failed";
+    return SSQ_EDEVICE;
+  }
+  *dc = (int64_t*)H->in_colptr.p;
+  *dr = (int32_t*)H->in_rows.p;
+  *dd = (int32_t*)H->in_dims.p;
+  *dp = (int64_t*)H->out_pairs.p;
+  *de = (int64_t*)H->out_ess.p;
+  return SSQ_OK;
+}
+
+static int check_matrix(const ssq_matrix* m) {
+  if (!m || m->n < 0 || m->nnz < 0 || (m->n > 0 && (!m->col_ptr || !m->dims))) return SSQ_EINVAL;
+  if (m->n == 0) return SSQ_OK;
+  if (m->col_ptr[0] != 0 || m->col_ptr[m->n] != m->nnz || (m->nnz > 0 && !m->rows)) return SSQ_EFORMAT;
+  for (int64_t c = 0; c < m->n; c++) {
+    if (m->col_ptr[c + 1] < m->col_ptr[c]) return SSQ_EFORMAT;
+    for (int64_t k = m->col_ptr[c]; k < m->col_ptr[c + 1]; k++) {
+      const int32_t r = m->rows[k];
+      if (r < 0 || r >= m->n || (k > m->col_ptr[c] && m->rows[k - 1] >= r)) return SSQ_EFORMAT;
+    }
+  }
+  return SSQ_OK;
+}
+
+// reduce host CSC arrays; pairs returned in the caller's indices
+static int reduce_host(Handle* H, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows,
+                       const int32_t* dims, const ssq_params& p, std::vector<int64_t>& pairs,
+                       std::vector<int64_t>& ess, ssq_stats* st) {
+  int64_t *dc, *dp, *de;
+  int32_t *dr, *dd;
+  int rc = upload(H, colptr, rows, dims, n, nnz, &dc, &dr, &dd, &dp, &de);
+  if (rc) return rc;
+  int64_t np = 0, ne = 0;
+  ssq_params q = p;
+  q.dualize = 0;
+  rc = H->eng.reduce_device(n, nnz, dc, dr, dd, q, dp, &np, st, de, &ne);
+  if (rc) {
+    H->err = H->eng.error();
+    return rc;
+  }
+  pairs.resize((size_t)(2 * np));
+  ess.resize((size_t)ne);
+  hipStream_t s = H->eng.stream();
+  if ((np && hipMemcpyAsync(pairs.data(), dp, sizeof(int64_t) * 2 * np, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (ne && hipMemcpyAsync(ess.data(), de, sizeof(int64_t) * ne, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    H->err = "device to host copy failed";
+    return SSQ_EDEVICE;
+  }
+  return SSQ_OK;
+}
+
+static int fill_pairs(const std::vector<int64_t>& pairs, const std::vector<int64_t>& ess, ssq_pairs* out) {
+  out->np = (int64_t)pairs.size() / 2;
+  out->ne = (int64_t)ess.size();
+  out->pairs = (int64_t*)std::malloc(sizeof(int64_t) * std::max<size_t>(pairs.size(), 1));
+  out->essential = (int64_t*)std::malloc(sizeof(int64_t) * std::max<size_t>(ess.size(), 1));
+  if (!out->pairs || !out->essential) {
+    ssq_pairs_free(out);
+    return SSQ_ENOMEM;
+  }
+  if (!pairs.empty()) std::memcpy(out->pairs, pairs.data(), sizeof(int64_t) * pairs.size());
+  if (!ess.empty()) std::memcpy(out->essential, ess.data(), sizeof(int64_t) * ess.size());
+  return SSQ_OK;
+}
+
+void dualize_matrix(const ssq_matrix& in, std::vector<int64_t>& colptr, std::vector<int32_t>& rows,
+                    std::vector<int32_t>& dims) {
+  const int64_t n = in.n;
+  int32_t maxd = 0;
+  for (int64_t c = 0; c < n; c++) maxd = std::max(maxd, in.dims[c]);
+  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  for (int64_t k = 0; k < in.nnz; k++) cnt[(size_t)(n - 1 - in.rows[k]) + 1]++;
+  for (int64_t c = 0; c < n; c++) cnt[c + 1] += cnt[c];
+  colptr = cnt;
+  rows.assign((size_t)in.nnz, 0);
+  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+  // visiting original columns from high to low gives ascending dual rows per column
+  for (int64_t c = n - 1; c >= 0; c--)
+    for (int64_t k = in.col_ptr[c]; k < in.col_ptr[c + 1]; k++) {
+      const int64_t dc = n - 1 - in.rows[k];
+      rows[(size_t)fill[dc]++] = (int32_t)(n - 1 - c);
+    }
+  dims.resize((size_t)n);
+  for (int64_t c = 0; c < n; c++) dims[(size_t)(n - 1 - c)] = maxd - in.dims[c];
+}
+
+}  // namespace ssq
+
+using namespace ssq;
+
+struct ssq_handle {
+  Handle* h;
+};
+
+extern "C" int ssq_create(int device, ssq_handle** out) {
+  if (!out) return SSQ_EINVAL;
+  *out = nullptr;
+  Handle* H = new (std::nothrow) Handle(device);
+  if (!H) return SSQ_ENOMEM;
+  int rc = H->eng.init();
+  if (rc) {
+    delete H;
+    return rc;
+  }
+  ssq_handle* h = new (std::nothrow) ssq_handle{H};
+  if (!h) {
+    delete H;
+    return SSQ_ENOMEM;
+  }
+  *out = h;
+  return SSQ_OK;
+}
+
+extern "C" int ssq_destroy(ssq_handle* h) {
+  if (!h) return SSQ_EINVAL;
+  for (DevBuf* b : {&h->h->in_colptr, &h->h->in_rows, &h->h->in_dims, &h->h->out_pairs, &h->h->out_ess})
+    if (b->p) (void)hipFree(b->p);
+  delete h->h;
+  delete h;
+  return SSQ_OK;
+}
+
+extern "C" const char* ssq_last_error(const ssq_handle* h) { return h ? h->h->err.c_str() : "null handle"; }
+
+extern "C" int ssq_reduce(ssq_handle* h, const ssq_matrix* m, const ssq_params* p, ssq_pairs* out,
+                          ssq_stats* stats) {
+  if (!h || !p || !out) return SSQ_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  h->h->err.clear();
+  int rc = check_matrix(m);
+  if (rc) {
+    h->h->err = "malformed matrix";
+    return rc;
+  }
+  std::vector<int64_t> pairs, ess;
+  if (p->dualize == 1) {
+    // cohomology: reduce the anti-transpose; pair (b', d') maps back to (n-1-d', n-1-b')
+    // (specseq.h:303-363,598-600,1188)
+    std::vector<int64_t> cp;
+    std::vector<int32_t> rw, dm;
+    dualize_matrix(*m, cp, rw, dm);
+    rc = reduce_host(h->h, m->n, m->nnz, cp.data(), rw.data(), dm.data(), *p, pairs, ess, stats);
+    if (rc) return rc;
+    const int64_t n = m->n, np = (int64_t)pairs.size() / 2;
+    std::vector<int64_t> death_of((size_t)n, -1);
+    for (int64_t i = 0; i < np; i++) death_of[(size_t)(n - 1 - pairs[2 * i + 1])] = n - 1 - pairs[2 * i];
+    pairs.clear();
+    for (int64_t b = 0; b < n; b++)
+      if (death_of[(size_t)b] >= 0) {
+        pairs.push_back(b);
+        pairs.push_back(death_of[(size_t)b]);
+      }
+    for (auto& e : ess) e = n - 1 - e;
+    std::reverse(ess.begin(), ess.end());
+  } else if (p->dualize == 0) {
+    rc = reduce_host(h->h, m->n, m->nnz, m->col_ptr, m->rows, m->dims, *p, pairs, ess, stats);
+    if (rc) return rc;
+  } else {
+    h->h->err = "dualize must be 0 or 1";
+    return SSQ_EINVAL;
+  }
+  return fill_pairs(pairs, ess, out);
+}
+
+extern "C" int ssq_reduce_device(ssq_handle* h, int64_t n, int64_t nnz, const int64_t* d_col_ptr,
+                                 const int32_t* d_rows, const int32_t* d_dims, const ssq_params* p,
+                                 int64_t* d_pairs_out, int64_t* np_out, ssq_stats* stats) {
+  if (!h || !p) return SSQ_EINVAL;
+  h->h->err.clear();
+  if (p->dualize) {
+    h->h->err = "dualize is supported by ssq_reduce (host matrices) only";
+    return SSQ_EINVAL;
+  }
+  int rc = h->h->eng.reduce_device(n, nnz, d_col_ptr, d_rows, d_dims, *p, d_pairs_out, np_out, stats);
+  if (rc) h->h->err = h->h->eng.error();
+  return rc;
+}
+
+extern "C" int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, int count, const ssq_params* p,
+                                ssq_pairs* outs, ssq_stats* stats) {
+  if (!h || !p || !outs || count < 0 || (count > 0 && !mats)) return SSQ_EINVAL;
+  h->h->err.clear();
+  if (p->dualize) {
+    h->h->err = "dualize is not supported in batches";
+    return SSQ_EINVAL;
+  }
+  // block-diagonal layout: complex i's cells are shifted by base[i]
+  std::vector<int64_t> base((size_t)count + 1, 0), nzb((size_t)count + 1, 0);
+  for (int i = 0; i < count; i++) {
+    std::memset(&outs[i], 0, sizeof(ssq_pairs));
+    int rc = check_matrix(mats[i]);
+    if (rc) {
+      h->h->err = "malformed matrix " + std::to_string(i);
+      return rc;
+    }
+    base[i + 1] = base[i] + mats[i]->n;
+    nzb[i + 1] = nzb[i] + mats[i]->nnz;
+  }
+  const int64_t N = base[count], NNZ = nzb[count];
+  if (N >= 0x7fffffffLL) {
+    h->h->err = "batch exceeds 2^31 cells";
+    return SSQ_EINVAL;
+  }
+  std::vector<int64_t> cp((size_t)N + 1);
+  std::vector<int32_t> rw((size_t)NNZ), dm((size_t)N);
+  for (int i = 0; i < count; i++) {
+    const ssq_matrix* m = mats[i];
+    for (int64_t c = 0; c <= m->n; c++) cp[(size_t)(base[i] + c)] = nzb[i] + m->col_ptr[c];
+    for (int64_t k = 0; k < m->nnz; k++) rw[(size_t)(nzb[i] + k)] = (int32_t)(m->rows[k] + base[i]);
+    if (m->n) std::memcpy(&dm[(size_t)base[i]], m->dims, sizeof(int32_t) * m->n);
+  }
+  std::vector<int64_t> pairs, ess;
+  int rc = reduce_host(h->h, N, NNZ, cp.data(), rw.data(), dm.data(), *p, pairs, ess, stats);
+  if (rc) return rc;
+  // pairs are sorted by birth, so each complex's pairs are a contiguous run
+  size_t pi = 0, ei = 0;
+  for (int i = 0; i < count; i++) {
+    std::vector<int64_t> pp, ee;
+    while (pi < pairs.size() && pairs[pi] < base[i + 1]) {
+      pp.push_back(pairs[pi] - base[i]);
+      pp.push_back(pairs[pi + 1] - base[i]);
+      pi += 2;
+    }
+    while (ei < ess.size() && ess[ei] < base[i + 1]) ee.push_back(ess[ei++] - base[i]);
+    rc = fill_pairs(pp, ee, &outs[i]);
+    if (rc) return rc;
+  }
+  return SSQ_OK;
+}

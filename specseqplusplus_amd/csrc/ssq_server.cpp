@@ -1,0 +1,266 @@
+// Continuous-batching request server on one GPU (new; no reference counterpart).
+//
+// Requests (boundary matrices, or point clouds that are turned into Vietoris-Rips matrices)
+// enter one queue.  Each worker owns an engine handle with its own HIP stream.  A worker
+// that is free takes the oldest requests that fit both max_batch and the shared budget of
+// estimated device bytes, reduces them as one block-diagonal batch, publishes each result
+// and returns the budget.  With several workers, a batch is formed and launched while other
+// batches are still on the device, so requests that arrive during a batch do not wait for it
+// to finish.  Each result is the pair set of its own matrix: block-diagonal batching never
+// mixes rows of different requests.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ssq_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t id;
+  Clock::time_point t_submit;
+  std::vector<int64_t> colptr;
+  std::vector<int32_t> rows, dims;
+  std::vector<float> pts;
+  int64_t npts = 0;
+  int pdim = 0, maxdim = 0;
+  float r = 0;
+  bool is_points = false;
+  int64_t est_bytes = 0;
+};
+
+struct Result {
+  std::vector<int64_t> pairs, ess;
+  double latency_ms = 0;
+  int status = SSQ_OK;
+};
+
+int64_t estimate_bytes(int64_t n, int64_t nnz) {
+  // input copy + column store arena (4x nnz) + per-cell state (~48 B) + pairs
+  return 24 * nnz + 64 * n + (1 << 20);
+}
+
+int64_t estimate_points(int64_t npts, int maxdim) {
+  // unknown clique counts until the matrix is built: budget a dense-ish complex of degree 32
+  const int64_t e = npts * 16, t = maxdim >= 2 ? npts * 160 : 0;
+  return estimate_bytes(npts + e + t, 2 * e + 3 * t);
+}
+
+}  // namespace
+
+struct ssq_server {
+  ssq_server_config cfg;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::unordered_map<int64_t, Result> done;
+  std::vector<std::thread> workers;
+  int64_t next_id = 1;
+  int64_t in_flight_bytes = 0;
+  int64_t outstanding = 0;
+  bool stop = false;
+  int init_status = SSQ_OK;
+
+  void worker_main(int wid) {
+    ssq_handle* h = nullptr;
+    int rc = ssq_create(cfg.device, &h);
+    if (rc) {
+      std::lock_guard<std::mutex> g(mu);
+      init_status = rc;
+    }
+    for (;;) {
+      std::vector<Request*> batch;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+        if (stop && queue.empty()) break;
+        // admission: oldest first while it fits the batch size and the device budget; the
+        // first request is always admitted when nothing else is in flight
+        while (!queue.empty() && (int)batch.size() < cfg.max_batch) {
+          Request* q = queue.front();
+          if (in_flight_bytes + q->est_bytes > cfg.mem_budget_bytes && (in_flight_bytes > 0 || !batch.empty()))
+            break;
+          queue.pop_front();
+          in_flight_bytes += q->est_bytes;
+          batch.push_back(q);
+        }
+        if (batch.empty()) {
+          // budget full: wait for another worker to return budget
+          cv_done.wait(lk);
+          continue;
+        }
+      }
+      std::vector<ssq_matrix> mats(batch.size());
+      std::vector<int> st(batch.size(), SSQ_OK);
+      for (size_t i = 0; i < batch.size(); i++) {
+        Request* q = batch[i];
+        if (q->is_points) {
+          st[i] = ssq::vr_build(q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &mats[i]);
+        } else {
+          mats[i].n = (int64_t)q->dims.size();
+          mats[i].nnz = (int64_t)q->rows.size();
+          mats[i].col_ptr = q->colptr.data();
+          mats[i].rows = q->rows.data();
+          mats[i].dims = q->dims.data();
+        }
+      }
+      std::vector<const ssq_matrix*> ptrs;
+      std::vector<size_t> idx;
+      for (size_t i = 0; i < batch.size(); i++)
+        if (st[i] == SSQ_OK) {
+          ptrs.push_back(&mats[i]);
+          idx.push_back(i);
+        }
+      std::vector<ssq_pairs> outs(ptrs.size());
+      int brc = SSQ_OK;
+      if (!ptrs.empty()) {
+        brc = h ? ssq_batch_reduce(h, ptrs.data(), (int)ptrs.size(), &cfg.params, outs.data(), nullptr) : SSQ_EDEVICE;
+      }
+      const auto t_end = Clock::now();
+      {
+        std::lock_guard<std::mutex> g(mu);
+        for (size_t i = 0; i < batch.size(); i++) {
+          Result& R = done[batch[i]->id];
+          R.latency_ms = std::chrono::duration<double, std::milli>(t_end - batch[i]->t_submit).count();
+          R.status = st[i];
+        }
+        for (size_t k = 0; k < idx.size(); k++) {
+          Result& R = done[batch[idx[k]]->id];
+          if (brc) {
+            R.status = brc;
+            continue;
+          }
+          R.pairs.assign(outs[k].pairs, outs[k].pairs + 2 * outs[k].np);
+          R.ess.assign(outs[k].essential, outs[k].essential + outs[k].ne);
+          ssq_pairs_free(&outs[k]);
+        }
+        for (Request* q : batch) {
+          in_flight_bytes -= q->est_bytes;
+          outstanding--;
+        }
+      }
+      for (size_t i = 0; i < batch.size(); i++)
+        if (batch[i]->is_points && st[i] == SSQ_OK) ssq_matrix_free(&mats[i]);
+      for (Request* q : batch) delete q;
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+    if (h) ssq_destroy(h);
+    (void)wid;
+  }
+
+  int submit(Request* q, int64_t* ticket) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      if (stop) {
+        delete q;
+        return SSQ_EINVAL;
+      }
+      q->id = next_id++;
+      q->t_submit = Clock::now();
+      queue.push_back(q);
+      outstanding++;
+      *ticket = q->id;
+    }
+    cv_work.notify_one();
+    return SSQ_OK;
+  }
+};
+
+extern "C" int ssq_server_create(const ssq_server_config* cfg, ssq_server** out) {
+  if (!cfg || !out || cfg->streams < 1 || cfg->max_batch < 1) return SSQ_EINVAL;
+  *out = nullptr;
+  ssq_server* s = new (std::nothrow) ssq_server();
+  if (!s) return SSQ_ENOMEM;
+  s->cfg = *cfg;
+  if (s->cfg.mem_budget_bytes <= 0) {
+    size_t fr = 0, tot = 0;
+    if (hipSetDevice(cfg->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      delete s;
+      return SSQ_EDEVICE;
+    }
+    s->cfg.mem_budget_bytes = (int64_t)(fr / 2);
+  }
+  for (int w = 0; w < cfg->streams; w++) s->workers.emplace_back(&ssq_server::worker_main, s, w);
+  *out = s;
+  return SSQ_OK;
+}
+
+extern "C" int ssq_server_submit_matrix(ssq_server* s, const ssq_matrix* m, int64_t* ticket) {
+  if (!s || !m || !ticket || m->n < 0 || (m->n && (!m->col_ptr || !m->dims))) return SSQ_EINVAL;
+  Request* q = new (std::nothrow) Request();
+  if (!q) return SSQ_ENOMEM;
+  q->colptr.assign(m->col_ptr, m->col_ptr + m->n + 1);
+  if (m->nnz) q->rows.assign(m->rows, m->rows + m->nnz);
+  q->dims.assign(m->dims, m->dims + m->n);
+  q->est_bytes = estimate_bytes(m->n, m->nnz);
+  return s->submit(q, ticket);
+}
+
+extern "C" int ssq_server_submit_points(ssq_server* s, const float* pts, int64_t npts, int pdim, float r, int maxdim,
+                                        int64_t* ticket) {
+  if (!s || !ticket || npts < 0 || pdim < 1 || (npts && !pts) || maxdim < 0 || maxdim > 3) return SSQ_EINVAL;
+  Request* q = new (std::nothrow) Request();
+  if (!q) return SSQ_ENOMEM;
+  q->is_points = true;
+  q->pts.assign(pts, pts + npts * pdim);
+  q->npts = npts;
+  q->pdim = pdim;
+  q->r = r;
+  q->maxdim = maxdim;
+  q->est_bytes = estimate_points(npts, maxdim);
+  return s->submit(q, ticket);
+}
+
+extern "C" int ssq_server_poll(ssq_server* s, int64_t ticket, ssq_pairs* out, double* latency_ms) {
+  if (!s || !out) return SSQ_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->done.find(ticket);
+  if (it == s->done.end()) return SSQ_EBUSY;
+  Result R = std::move(it->second);
+  s->done.erase(it);
+  if (latency_ms) *latency_ms = R.latency_ms;
+  if (R.status) return R.status;
+  out->np = (int64_t)R.pairs.size() / 2;
+  out->ne = (int64_t)R.ess.size();
+  out->pairs = (int64_t*)std::malloc(sizeof(int64_t) * std::max<size_t>(1, R.pairs.size()));
+  out->essential = (int64_t*)std::malloc(sizeof(int64_t) * std::max<size_t>(1, R.ess.size()));
+  if (!out->pairs || !out->essential) {
+    ssq_pairs_free(out);
+    return SSQ_ENOMEM;
+  }
+  if (!R.pairs.empty()) std::memcpy(out->pairs, R.pairs.data(), sizeof(int64_t) * R.pairs.size());
+  if (!R.ess.empty()) std::memcpy(out->essential, R.ess.data(), sizeof(int64_t) * R.ess.size());
+  return SSQ_OK;
+}
+
+extern "C" int ssq_server_wait_all(ssq_server* s) {
+  if (!s) return SSQ_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+  return s->init_status;
+}
+
+extern "C" int ssq_server_destroy(ssq_server* s) {
+  if (!s) return SSQ_EINVAL;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  s->cv_done.notify_all();
+  for (auto& t : s->workers) t.join();
+  for (Request* q : s->queue) delete q;
+  delete s;
+  return SSQ_OK;
+}

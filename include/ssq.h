@@ -74,6 +74,9 @@ typedef struct ssq_stats {
   int64_t n_additions;   /* column additions performed                                 */
   double add_bytes;      /* algorithmic bytes of those additions: 4(|a|+|b|+|a^b|)+24 */
   int64_t store_bytes;   /* device bytes reserved for the column store                 */
+  double t_add_kernel_ms; /* summed duration of the column-addition kernel launches, from
+                             HIP events on the engine stream (SSQ_KERNEL_TIMING=1), else 0 */
+  int64_t n_add_launches; /* column-addition kernel launches                            */
 } ssq_stats;
 
 /* ---- I/O (main.cu:90-167 processFiles; main.cu:191-229 writeVectorToFile) ---------- */
@@ -103,6 +106,15 @@ int ssq_reduce(ssq_handle* h, const ssq_matrix* m, const ssq_params* p, ssq_pair
 int ssq_reduce_device(ssq_handle* h, int64_t n, int64_t nnz, const int64_t* d_col_ptr,
                       const int32_t* d_rows, const int32_t* d_dims, const ssq_params* p,
                       int64_t* d_pairs_out, int64_t* np_out, ssq_stats* stats);
+
+/* ---- Device memory plumbing (no reference counterpart; the reference keeps its inputs in
+ * managed memory, specseq.h:148-270).  For callers that keep matrices resident in HBM and
+ * call ssq_reduce_device; all copies run on the handle's stream and are complete on return. */
+int ssq_device_alloc(ssq_handle* h, size_t bytes, void** dptr);
+int ssq_device_free(ssq_handle* h, void* dptr);
+int ssq_copy_to_device(ssq_handle* h, void* dst, const void* src, size_t bytes);
+int ssq_copy_to_host(ssq_handle* h, void* dst, const void* src, size_t bytes);
+int ssq_device_synchronize(ssq_handle* h);
 
 /* ---- Batching (new; no reference counterpart): many complexes, one engine pass. ---- */
 /* The matrices are laid out block-diagonally on the device and reduced together; outs[i]

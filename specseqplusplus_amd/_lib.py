@@ -34,7 +34,8 @@ class ssq_stats(ctypes.Structure):
                 ("t_collapse_ms", ctypes.c_double), ("n_pending", ctypes.c_int64), ("n_apparent", ctypes.c_int64),
                 ("n_cleared_hd", ctypes.c_int64), ("n_kept_edges", ctypes.c_int64), ("n_pages", ctypes.c_int64),
                 ("n_rounds", ctypes.c_int64), ("n_additions", ctypes.c_int64), ("add_bytes", ctypes.c_double),
-                ("store_bytes", ctypes.c_int64)]
+                ("store_bytes", ctypes.c_int64), ("t_add_kernel_ms", ctypes.c_double),
+                ("n_add_launches", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -71,6 +72,11 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_double)]),
     "ssq_server_wait_all": (ctypes.c_int, [ctypes.c_void_p]),
     "ssq_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "ssq_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "ssq_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ssq_copy_to_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "ssq_copy_to_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "ssq_device_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "ssq_vr_build": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
 }

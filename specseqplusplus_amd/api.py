@@ -123,6 +123,31 @@ class Handle:
                                           ctypes.byref(np_), ctypes.byref(st)), self.error())
         return np_.value, st.as_dict()
 
+    # ---- device-resident buffers (HBM), owned by the caller through this handle ----
+    def device_array(self, arr):
+        """Copy a numpy array into a new device buffer; returns a DeviceArray."""
+        arr = np.ascontiguousarray(arr)
+        d = DeviceArray(self, arr.nbytes, arr.dtype, arr.shape)
+        L.check(L.lib().ssq_copy_to_device(self._h, d.ptr, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes),
+                self.error())
+        return d
+
+    def device_empty(self, shape, dtype):
+        dt = np.dtype(dtype)
+        return DeviceArray(self, int(np.prod(shape)) * dt.itemsize, dt, shape)
+
+    def synchronize(self):
+        L.check(L.lib().ssq_device_synchronize(self._h), self.error())
+
+    def upload(self, m):
+        """Matrix resident in HBM: (n, nnz, colptr, rows, dims) device arrays."""
+        return DeviceMatrix(m.n, m.nnz, self.device_array(m.colptr), self.device_array(m.rows if m.nnz else
+                            np.zeros(1, np.int32)), self.device_array(m.dims if m.n else np.zeros(1, np.int32)))
+
+    def reduce_resident(self, dm, d_pairs, p=None):
+        """ssq_reduce_device on an uploaded matrix; returns (n_pairs, stats)."""
+        return self.reduce_device(dm.n, dm.nnz, dm.colptr.ptr, dm.rows.ptr, dm.dims.ptr, d_pairs.ptr, p)
+
     def batch_reduce(self, mats, p=None, stats=False):
         p = p or params()
         structs = [m.c_struct() for m in mats]
@@ -133,6 +158,40 @@ class Handle:
                 self.error())
         res = [_pairs_from_c(outs[i]) for i in range(len(mats))]
         return (res, st.as_dict()) if stats else res
+
+
+class DeviceArray:
+    """A device (HBM) buffer allocated through libssq; freed when collected."""
+
+    def __init__(self, handle, nbytes, dtype, shape):
+        self.handle, self.nbytes, self.dtype, self.shape = handle, int(nbytes), np.dtype(dtype), tuple(np.atleast_1d(shape))
+        p = ctypes.c_void_p()
+        L.check(L.lib().ssq_device_alloc(handle._h, max(self.nbytes, 1), ctypes.byref(p)), "ssq_device_alloc")
+        self.ptr = p.value
+
+    def numpy(self, count=None):
+        n = int(np.prod(self.shape)) if count is None else int(count)
+        out = np.empty(n, dtype=self.dtype)
+        if n:
+            L.check(L.lib().ssq_copy_to_host(self.handle._h, out.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                                            n * self.dtype.itemsize), "ssq_copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr and self.handle._h:
+            L.lib().ssq_device_free(self.handle._h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceMatrix:
+    def __init__(self, n, nnz, colptr, rows, dims):
+        self.n, self.nnz, self.colptr, self.rows, self.dims = n, nnz, colptr, rows, dims
 
 
 _default = None

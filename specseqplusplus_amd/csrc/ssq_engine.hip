@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -481,6 +482,181 @@ __global__ void k_twist(Dev d, const int32_t* cols, int32_t ncols) {
   }
 }
 
+
+// ------------------------------------------------------------------------------------
+// dualize (cohomology): the anti-transpose.  Dual cell n-1-c has boundary
+// {n-1-x : c is a facet of x} and dimension maxdim - dim(c) (specseq.h:303-363,
+// kernel.h:521-580); its pairs map back as (b', d') -> (n-1-d', n-1-b').
+// ------------------------------------------------------------------------------------
+__global__ void k_dual_count(int64_t n, const int64_t* colptr, const int32_t* rows, const uint8_t* removed,
+                             int32_t* cnt) {
+  GRID_STRIDE(c, n) {
+    if (removed[c]) continue;
+    for (int64_t k = colptr[c]; k < colptr[c + 1]; k++) atomicAdd(&cnt[n - 1 - rows[k]], 1);
+  }
+}
+
+__global__ void k_dual_fill(int64_t n, const int64_t* colptr, const int32_t* rows, const uint8_t* removed,
+                            const int64_t* dcolptr, int32_t* fill, int32_t* drows) {
+  GRID_STRIDE(c, n) {
+    if (removed[c]) continue;
+    for (int64_t k = colptr[c]; k < colptr[c + 1]; k++) {
+      const int64_t dc = n - 1 - rows[k];
+      const int32_t pos = atomicAdd(&fill[dc], 1);
+      drows[dcolptr[dc] + pos] = (int32_t)(n - 1 - c);
+    }
+  }
+}
+
+__global__ void k_dual_dims(int64_t n, const int32_t* dims, int32_t maxd, const uint8_t* removed, int32_t* ddims,
+                            uint8_t* dremoved) {
+  GRID_STRIDE(c, n) {
+    ddims[n - 1 - c] = maxd - dims[c];
+    dremoved[n - 1 - c] = removed[c];
+  }
+}
+
+// Sort every segment v[off[s] : off[s+1]) ascending: one wavefront per segment; bitonic in
+// registers up to 64 entries, in LDS up to kSortLds, heapsort by one lane beyond (rare).
+constexpr int kSortLds = 2048;
+__device__ void heapsort_i32(int32_t* a, int64_t n) {
+  auto sift = [&](int64_t i, int64_t m) {
+    for (;;) {
+      int64_t l = 2 * i + 1, big = i;
+      if (l < m && a[l] > a[big]) big = l;
+      if (l + 1 < m && a[l + 1] > a[big]) big = l + 1;
+      if (big == i) return;
+      const int32_t t = a[i];
+      a[i] = a[big];
+      a[big] = t;
+      i = big;
+    }
+  };
+  for (int64_t i = n / 2 - 1; i >= 0; i--) sift(i, n);
+  for (int64_t m = n - 1; m > 0; m--) {
+    const int32_t t = a[0];
+    a[0] = a[m];
+    a[m] = t;
+    sift(0, m);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_segments(int64_t nseg, const int64_t* off, int32_t* v) {
+  __shared__ int32_t lds[4][kSortLds];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  int32_t* buf = lds[w];
+  for (int64_t s = (int64_t)blockIdx.x * 4 + w; s < nseg; s += (int64_t)gridDim.x * 4) {
+    const int64_t b = off[s];
+    const int64_t len = off[s + 1] - b;
+    if (len <= 1) continue;
+    if (len <= kWave) {
+      int32_t x = lane < len ? v[b + lane] : 0x7fffffff;
+      for (int k = 2; k <= kWave; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          const int32_t y = __shfl_xor(x, j, kWave);
+          const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+          x = (lower == up) ? min(x, y) : max(x, y);
+        }
+      if (lane < len) v[b + lane] = x;
+    } else if (len <= kSortLds) {
+      int P = kWave;
+      while (P < len) P <<= 1;
+      for (int i = lane; i < P; i += kWave) buf[i] = i < len ? v[b + i] : 0x7fffffff;
+      wave_sync();
+      for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int i = lane; i < P; i += kWave) {
+            const int l = i ^ j;
+            if (l > i) {
+              const int32_t a = buf[i], c = buf[l];
+              const bool asc = (i & k) == 0;
+              if ((a > c) == asc) {
+                buf[i] = c;
+                buf[l] = a;
+              }
+            }
+          }
+          wave_sync();
+        }
+      for (int i = lane; i < len; i += kWave) v[b + i] = buf[i];
+      wave_sync();
+    } else if (lane == 0) {
+      heapsort_i32(v + b, len);
+    }
+  }
+}
+
+// exclusive scan of int32 counts into int64 offsets out[0..n] (out[n] = total)
+constexpr int kScanBlock = 1024;
+__global__ void k_scan_partial(const int32_t* in, int64_t n, int64_t* bsum) {
+  __shared__ int64_t s[256];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
+  int64_t acc = 0;
+  for (int t = threadIdx.x; t < kScanBlock; t += 256)
+    if (b0 + t < n) acc += in[b0 + t];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+
+__global__ void k_scan_bsums(int64_t nb, int64_t* bsum) {
+  __shared__ int64_t part[1024];
+  const int64_t per = (nb + blockDim.x - 1) / blockDim.x;
+  const int64_t a = (int64_t)threadIdx.x * per;
+  const int64_t e = a + per < nb ? a + per : nb;
+  int64_t sum = 0;
+  for (int64_t b = a; b < e; b++) sum += bsum[b];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (unsigned t = 0; t < blockDim.x; t++) {
+      const int64_t v = part[t];
+      part[t] = run;
+      run += v;
+    }
+    bsum[nb] = run;
+  }
+  __syncthreads();
+  int64_t run = part[threadIdx.x];
+  for (int64_t b = a; b < e; b++) {
+    const int64_t v = bsum[b];
+    bsum[b] = run;
+    run += v;
+  }
+}
+
+__global__ void k_scan_final(const int32_t* in, int64_t n, const int64_t* bsum, int64_t nb, int64_t* out) {
+  __shared__ int64_t s[kScanBlock];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
+  for (int t = threadIdx.x; t < kScanBlock; t += 256) s[t] = (b0 + t < n) ? in[b0 + t] : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = bsum[blockIdx.x];
+    for (int t = 0; t < kScanBlock; t++) {
+      const int64_t v = s[t];
+      s[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kScanBlock; t += 256)
+    if (b0 + t < n) out[b0 + t] = s[t];
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+__global__ void k_dual_pairs(int64_t np, const int64_t* dpairs, int64_t n, int32_t* death_of) {
+  GRID_STRIDE(i, np) death_of[n - 1 - dpairs[2 * i + 1]] = (int32_t)(n - 1 - dpairs[2 * i]);
+}
+
+__global__ void k_dual_essential(int64_t ne, const int64_t* in, int64_t n, int64_t* out) {
+  GRID_STRIDE(k, ne) out[ne - 1 - k] = n - 1 - in[k];
+}
+
 // ------------------------------------------------------------------------------------
 // edge collapse support (model bit 1)
 // ------------------------------------------------------------------------------------
@@ -765,11 +941,16 @@ class Engine {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_));
     cus_ = prop.multiProcessorCount;
+    const char* kt = std::getenv("SSQ_KERNEL_TIMING");
+    timing_ = kt && kt[0] == '1';
+    if (timing_)
+      for (auto& e : ev_) HIPCHK(hipEventCreate(&e));
     return SSQ_OK;
   }
 
   const std::string& error() const { return err_; }
   hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
 
   int reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
                     const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
@@ -798,9 +979,11 @@ class Engine {
   void release() {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
                       &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
-                      &b_hash_, &b_hd_})
+                      &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
+    for (auto& e : ev_)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
   }
   int sync_read(const uint32_t* dsrc, int count) {
@@ -819,6 +1002,9 @@ class Engine {
                ssq_stats* st);
   int guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dims, const int64_t* d_colptr,
                    const int32_t* d_rows, const ssq_params& p, uint8_t** hd_out, ssq_stats* st);
+  int scan_counts(const int32_t* cnt, int64_t n, int64_t* out);
+  int build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
+                 const int32_t* dims, int64_t** dcolptr, int32_t** drows, int32_t** ddims);
 
   int device_ = 0;
   int cus_ = 256;
@@ -826,8 +1012,11 @@ class Engine {
   uint32_t* host_ctr_ = nullptr;
   std::string err_;
   DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
-      b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_;
+      b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
+      b_drem_, b_scan_;
   int64_t arena_hint_ = 0;
+  bool timing_ = false;
+  hipEvent_t ev_[8] = {};
 };
 
 template <typename KeyF, typename EmitF>
@@ -897,6 +1086,18 @@ struct KeyEssential {
 struct EmitI64 {
   int64_t* out;
   __device__ void operator()(int64_t i, int64_t pos) const { out[pos] = i; }
+};
+struct KeyDeathOf {
+  const int32_t* death_of;
+  __device__ int operator()(int64_t r) const { return death_of[r] >= 0 ? 0 : -1; }
+};
+struct EmitDeathOf {
+  const int32_t* death_of;
+  int64_t* out;
+  __device__ void operator()(int64_t r, int64_t pos) const {
+    out[2 * pos] = r;
+    out[2 * pos + 1] = death_of[r];
+  }
 };
 struct KeyVertex {
   const int32_t* dims;
@@ -1040,9 +1241,46 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
   return SSQ_OK;
 }
 
+
+int Engine::scan_counts(const int32_t* cnt, int64_t n, int64_t* out) {
+  const int64_t nb = std::max<int64_t>(1, (n + kScanBlock - 1) / kScanBlock);
+  int64_t* bsum;
+  int rc = ensure(b_scan_, (size_t)(nb + 1), &bsum);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)nb), dim3(256), 0, stream_, cnt, n, bsum);
+  hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, stream_, nb, bsum);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(256), 0, stream_, cnt, n, bsum, nb, out);
+  HIPCHK(hipGetLastError());
+  return SSQ_OK;
+}
+
+int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
+                       const int32_t* dims, int64_t** dcolptr, int32_t** drows, int32_t** ddims) {
+  int32_t* cnt;
+  uint8_t* drem;
+  int rc;
+  if ((rc = ensure(b_dcol_, (size_t)(n + 1), dcolptr)) || (rc = ensure(b_drow_, (size_t)std::max<int64_t>(nnz, 1), drows)) ||
+      (rc = ensure(b_ddim_, (size_t)n, ddims)) || (rc = ensure(b_drem_, (size_t)n, &drem)) ||
+      (rc = ensure(b_misc_, (size_t)n, &cnt)))
+    return rc;
+  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_dual_count, dim3(g), dim3(256), 0, stream_, n, colptr, rows, d.removed, cnt);
+  rc = scan_counts(cnt, n, *dcolptr);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
+  hipLaunchKernelGGL(k_dual_fill, dim3(g), dim3(256), 0, stream_, n, colptr, rows, d.removed, *dcolptr, cnt, *drows);
+  hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n, 4)), dim3(256), 0, stream_, n, *dcolptr, *drows);
+  hipLaunchKernelGGL(k_dual_dims, dim3(g), dim3(256), 0, stream_, n, dims, maxdim, d.removed, *ddims, drem);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(d.removed, drem, (size_t)n, hipMemcpyDeviceToDevice, stream_));
+  return SSQ_OK;
+}
+
 int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
                           const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
                           ssq_stats* st, int64_t* d_essential, int64_t* ne_out) {
+  // (d_colptr, d_rows, d_dims are re-pointed at the dual matrix when p.dualize is set)
   err_.clear();
   if (n < 0 || nnz < 0 || n >= 0x7fffffffLL || !np_out || (n > 0 && (!d_colptr || !d_dims || !d_pairs))) {
     err_ = "invalid arguments";
@@ -1107,16 +1345,33 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     if (rc) return rc;
   }
   uint8_t* hd = nullptr;
-  if ((p.model & 1) && maxdim >= 1) {
+  const bool dual = p.dualize == 1;
+  int64_t nnz_eff = nnz;
+  if (dual) {
+    // cohomology on the (collapsed) complex: reduce the anti-transpose; guided clearing is a
+    // statement about the top dimension of the primal complex and is not used here
+    int64_t* dcp;
+    int32_t *drw, *ddm;
+    rc = build_dual(d, n, nnz, maxdim, d_colptr, d_rows, d_dims, &dcp, &drw, &ddm);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(host_ctr_, dcp + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    std::memcpy(&nnz_eff, host_ctr_, sizeof(int64_t));
+    d_colptr = dcp;
+    d_rows = drw;
+    d_dims = ddm;
+    d.colptr = dcp;
+    d.dims = ddm;
+  } else if ((p.model & 1) && maxdim >= 1) {
     rc = guided_clear(d, n, maxdim, d_dims, d_colptr, d_rows, p, &hd, S);
     if (rc) return rc;
   }
 
   for (int attempt = 0;; attempt++) {
-    if ((rc = ensure(b_pool_, (size_t)(nnz + arena_cap), &d.pool))) return rc;
-    d.arena_base = nnz;
+    if ((rc = ensure(b_pool_, (size_t)(nnz_eff + arena_cap), &d.pool))) return rc;
+    d.arena_base = nnz_eff;
     d.arena_cap = arena_cap;
-    HIPCHK(hipMemcpyAsync(d.pool, d_rows, sizeof(int32_t) * std::max<int64_t>(nnz, 0), hipMemcpyDeviceToDevice,
+    HIPCHK(hipMemcpyAsync(d.pool, d_rows, sizeof(int32_t) * std::max<int64_t>(nnz_eff, 0), hipMemcpyDeviceToDevice,
                           stream_));
     HIPCHK(hipMemsetAsync(d.ctr, 0, 64 * sizeof(uint32_t), stream_));
     const int g = grid_for(n);
@@ -1196,11 +1451,20 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
           for (;;) {
             for (int t = 0; t < rounds_per_sync; t++, r++) {
               hipLaunchKernelGGL(k_claim, dim3(cg), dim3(256), 0, stream_, d, r);
+              if (timing_) HIPCHK(hipEventRecord(ev_[2 * t], stream_));
               hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(kWave * kAddWaves), 0, stream_, d, r, fl);
+              if (timing_) HIPCHK(hipEventRecord(ev_[2 * t + 1], stream_));
             }
             HIPCHK(hipGetLastError());
             rc = sync_read(d.ctr, 64);
             if (rc) return rc;
+            S->n_add_launches += rounds_per_sync;
+            if (timing_)
+              for (int t = 0; t < rounds_per_sync; t++) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev_[2 * t], ev_[2 * t + 1]));
+                S->t_add_kernel_ms += ms;
+              }
             S->n_rounds += rounds_per_sync;
             if (host_ctr_[44]) {
               oom = true;
@@ -1243,16 +1507,28 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     *np_out = np;
     if (d_essential && ne_out) {
       int64_t ne = 0;
-      rc = compact(n, 1, KeyEssential{d.piv, d.removed, d.state, d.low}, EmitI64{d_essential}, &ne);
+      int64_t* ess_dst = d_essential;
+      if (dual && (rc = ensure(b_keys_, (size_t)n, &ess_dst))) return rc;
+      rc = compact(n, 1, KeyEssential{d.piv, d.removed, d.state, d.low}, EmitI64{ess_dst}, &ne);
       if (rc) return rc;
+      if (dual && ne)
+        hipLaunchKernelGGL(k_dual_essential, dim3(grid_for(ne)), dim3(256), 0, stream_, ne, ess_dst, n, d_essential);
       *ne_out = ne;
+    }
+    if (dual) {  // (b', d') -> (n-1-d', n-1-b'), sorted by the new birth
+      int32_t* death_of = (int32_t*)d.leftone;
+      HIPCHK(hipMemsetAsync(death_of, 0xff, sizeof(int32_t) * n, stream_));
+      if (np) hipLaunchKernelGGL(k_dual_pairs, dim3(grid_for(np)), dim3(256), 0, stream_, np, d_pairs, n, death_of);
+      int64_t np2 = 0;
+      rc = compact(n, 1, KeyDeathOf{death_of}, EmitDeathOf{death_of, d_pairs}, &np2);
+      if (rc) return rc;
     }
     rc = sync_read(d.ctr, 64);
     if (rc) return rc;
     const unsigned long long* s64 = (const unsigned long long*)(host_ctr_ + 32);
     S->n_additions = (int64_t)s64[0];
     S->add_bytes = (double)s64[1];
-    S->store_bytes = (int64_t)(nnz + arena_cap) * 4;
+    S->store_bytes = (int64_t)(nnz_eff + arena_cap) * 4;
     arena_hint_ = std::max(arena_hint_, (int64_t)(*(const unsigned long long*)(host_ctr_ + 36)) * 2);
     auto T2 = std::chrono::steady_clock::now();
     S->t_reduce_ms = std::chrono::duration<double, std::milli>(T2 - T1).count();
@@ -1330,9 +1606,7 @@ static int reduce_host(Handle* H, int64_t n, int64_t nnz, const int64_t* colptr,
   int rc = upload(H, colptr, rows, dims, n, nnz, &dc, &dr, &dd, &dp, &de);
   if (rc) return rc;
   int64_t np = 0, ne = 0;
-  ssq_params q = p;
-  q.dualize = 0;
-  rc = H->eng.reduce_device(n, nnz, dc, dr, dd, q, dp, &np, st, de, &ne);
+  rc = H->eng.reduce_device(n, nnz, dc, dr, dd, p, dp, &np, st, de, &ne);
   if (rc) {
     H->err = H->eng.error();
     return rc;
@@ -1361,27 +1635,6 @@ static int fill_pairs(const std::vector<int64_t>& pairs, const std::vector<int64
   if (!pairs.empty()) std::memcpy(out->pairs, pairs.data(), sizeof(int64_t) * pairs.size());
   if (!ess.empty()) std::memcpy(out->essential, ess.data(), sizeof(int64_t) * ess.size());
   return SSQ_OK;
-}
-
-void dualize_matrix(const ssq_matrix& in, std::vector<int64_t>& colptr, std::vector<int32_t>& rows,
-                    std::vector<int32_t>& dims) {
-  const int64_t n = in.n;
-  int32_t maxd = 0;
-  for (int64_t c = 0; c < n; c++) maxd = std::max(maxd, in.dims[c]);
-  std::vector<int64_t> cnt((size_t)n + 1, 0);
-  for (int64_t k = 0; k < in.nnz; k++) cnt[(size_t)(n - 1 - in.rows[k]) + 1]++;
-  for (int64_t c = 0; c < n; c++) cnt[c + 1] += cnt[c];
-  colptr = cnt;
-  rows.assign((size_t)in.nnz, 0);
-  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-  // visiting original columns from high to low gives ascending dual rows per column
-  for (int64_t c = n - 1; c >= 0; c--)
-    for (int64_t k = in.col_ptr[c]; k < in.col_ptr[c + 1]; k++) {
-      const int64_t dc = n - 1 - in.rows[k];
-      rows[(size_t)fill[dc]++] = (int32_t)(n - 1 - c);
-    }
-  dims.resize((size_t)n);
-  for (int64_t c = 0; c < n; c++) dims[(size_t)(n - 1 - c)] = maxd - in.dims[c];
 }
 
 }  // namespace ssq
@@ -1433,32 +1686,8 @@ extern "C" int ssq_reduce(ssq_handle* h, const ssq_matrix* m, const ssq_params* 
     return rc;
   }
   std::vector<int64_t> pairs, ess;
-  if (p->dualize == 1) {
-    // cohomology: reduce the anti-transpose; pair (b', d') maps back to (n-1-d', n-1-b')
-    // (specseq.h:303-363,598-600,1188)
-    std::vector<int64_t> cp;
-    std::vector<int32_t> rw, dm;
-    dualize_matrix(*m, cp, rw, dm);
-    rc = reduce_host(h->h, m->n, m->nnz, cp.data(), rw.data(), dm.data(), *p, pairs, ess, stats);
-    if (rc) return rc;
-    const int64_t n = m->n, np = (int64_t)pairs.size() / 2;
-    std::vector<int64_t> death_of((size_t)n, -1);
-    for (int64_t i = 0; i < np; i++) death_of[(size_t)(n - 1 - pairs[2 * i + 1])] = n - 1 - pairs[2 * i];
-    pairs.clear();
-    for (int64_t b = 0; b < n; b++)
-      if (death_of[(size_t)b] >= 0) {
-        pairs.push_back(b);
-        pairs.push_back(death_of[(size_t)b]);
-      }
-    for (auto& e : ess) e = n - 1 - e;
-    std::reverse(ess.begin(), ess.end());
-  } else if (p->dualize == 0) {
-    rc = reduce_host(h->h, m->n, m->nnz, m->col_ptr, m->rows, m->dims, *p, pairs, ess, stats);
-    if (rc) return rc;
-  } else {
-    h->h->err = "dualize must be 0 or 1";
-    return SSQ_EINVAL;
-  }
+  rc = reduce_host(h->h, m->n, m->nnz, m->col_ptr, m->rows, m->dims, *p, pairs, ess, stats);
+  if (rc) return rc;
   return fill_pairs(pairs, ess, out);
 }
 
@@ -1467,10 +1696,6 @@ extern "C" int ssq_reduce_device(ssq_handle* h, int64_t n, int64_t nnz, const in
                                  int64_t* d_pairs_out, int64_t* np_out, ssq_stats* stats) {
   if (!h || !p) return SSQ_EINVAL;
   h->h->err.clear();
-  if (p->dualize) {
-    h->h->err = "dualize is supported by ssq_reduce (host matrices) only";
-    return SSQ_EINVAL;
-  }
   int rc = h->h->eng.reduce_device(n, nnz, d_col_ptr, d_rows, d_dims, *p, d_pairs_out, np_out, stats);
   if (rc) h->h->err = h->h->eng.error();
   return rc;
@@ -1480,10 +1705,6 @@ extern "C" int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, in
                                 ssq_pairs* outs, ssq_stats* stats) {
   if (!h || !p || !outs || count < 0 || (count > 0 && !mats)) return SSQ_EINVAL;
   h->h->err.clear();
-  if (p->dualize) {
-    h->h->err = "dualize is not supported in batches";
-    return SSQ_EINVAL;
-  }
   // block-diagonal layout: complex i's cells are shifted by base[i]
   std::vector<int64_t> base((size_t)count + 1, 0), nzb((size_t)count + 1, 0);
   for (int i = 0; i < count; i++) {
@@ -1525,5 +1746,49 @@ extern "C" int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, in
     rc = fill_pairs(pp, ee, &outs[i]);
     if (rc) return rc;
   }
+  return SSQ_OK;
+}
+
+extern "C" int ssq_device_alloc(ssq_handle* h, size_t bytes, void** dptr) {
+  if (!h || !dptr) return SSQ_EINVAL;
+  *dptr = nullptr;
+  if (hipSetDevice(h->h->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  if (hipMalloc(dptr, std::max<size_t>(bytes, 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    *dptr = nullptr;
+    h->h->err = "device allocation failed";
+    return SSQ_ENOMEM;
+  }
+  return SSQ_OK;
+}
+
+extern "C" int ssq_device_free(ssq_handle* h, void* dptr) {
+  if (!h) return SSQ_EINVAL;
+  if (dptr && hipFree(dptr) != hipSuccess) return SSQ_EDEVICE;
+  return SSQ_OK;
+}
+
+static int copy_sync(ssq_handle* h, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (!h || (bytes && (!dst || !src))) return SSQ_EINVAL;
+  if (!bytes) return SSQ_OK;
+  hipStream_t s = h->h->eng.stream();
+  if (hipMemcpyAsync(dst, src, bytes, kind, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    h->h->err = "copy failed";
+    return SSQ_EDEVICE;
+  }
+  return SSQ_OK;
+}
+
+extern "C" int ssq_copy_to_device(ssq_handle* h, void* dst, const void* src, size_t bytes) {
+  return copy_sync(h, dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+extern "C" int ssq_copy_to_host(ssq_handle* h, void* dst, const void* src, size_t bytes) {
+  return copy_sync(h, dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
+extern "C" int ssq_device_synchronize(ssq_handle* h) {
+  if (!h) return SSQ_EINVAL;
+  if (hipSetDevice(h->h->eng.device()) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return SSQ_EDEVICE;
   return SSQ_OK;
 }

@@ -18,8 +18,4 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
 // Vietoris-Rips boundary matrix (host builder).
 int vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim, ssq_matrix* out);
 
-// Anti-transpose for dualize (specseq.h:303-363): cell c -> n-1-c, dims reversed.
-void dualize_matrix(const ssq_matrix& in, std::vector<int64_t>& colptr, std::vector<int32_t>& rows,
-                    std::vector<int32_t>& dims);
-
 }  // namespace ssq

@@ -1,0 +1,57 @@
+"""Python handle on the libssq continuous-batching server (ssq_server_*)."""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .api import params as make_params
+
+
+class Server:
+    def __init__(self, device=0, streams=2, max_batch=64, mem_budget_bytes=0, p=None):
+        cfg = L.ssq_server_config()
+        cfg.device, cfg.streams, cfg.max_batch, cfg.mem_budget_bytes = device, streams, max_batch, mem_budget_bytes
+        cfg.params = p or make_params()
+        self._s = ctypes.c_void_p()
+        L.check(L.lib().ssq_server_create(ctypes.byref(cfg), ctypes.byref(self._s)), "ssq_server_create")
+
+    def submit_matrix(self, m):
+        t = ctypes.c_int64()
+        ms = m.c_struct()
+        L.check(L.lib().ssq_server_submit_matrix(self._s, ctypes.byref(ms), ctypes.byref(t)))
+        return t.value
+
+    def submit_points(self, pts, r, maxdim=2):
+        pts = np.ascontiguousarray(pts, dtype=np.float32)
+        t = ctypes.c_int64()
+        L.check(L.lib().ssq_server_submit_points(self._s, pts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                 pts.shape[0], pts.shape[1], float(r), int(maxdim), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        """(pairs, essential, latency_ms) or None while pending."""
+        pr = L.ssq_pairs()
+        lat = ctypes.c_double()
+        rc = L.lib().ssq_server_poll(self._s, ticket, ctypes.byref(pr), ctypes.byref(lat))
+        if rc == L.SSQ_EBUSY:
+            return None
+        L.check(rc, f"request {ticket}")
+        n = pr.np
+        pairs = np.ctypeslib.as_array(pr.pairs, shape=(max(2 * n, 1),))[:2 * n].copy().reshape(-1, 2)
+        ess = np.ctypeslib.as_array(pr.essential, shape=(max(pr.ne, 1),))[:pr.ne].copy()
+        L.lib().ssq_pairs_free(ctypes.byref(pr))
+        return pairs, ess, lat.value
+
+    def wait_all(self):
+        L.check(L.lib().ssq_server_wait_all(self._s), "wait_all")
+
+    def close(self):
+        if self._s:
+            L.lib().ssq_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -31,7 +31,11 @@ def vr_simplices(points, r, maxdim):
     """List of (filtration(float32), dim, vertex tuple) for the VR complex, sorted."""
     X = np.asarray(points, dtype=np.float64)
     n = len(X)
-    D = np.sqrt(((X[:, None, :] - X[None, :, :]) ** 2).sum(-1)).astype(np.float32)
+    acc = np.zeros((n, n))
+    for k in range(X.shape[1]):  # float64 sum in coordinate order, then one float32 rounding
+        diff = X[:, None, k] - X[None, :, k]
+        acc += diff * diff
+    D = np.sqrt(acc).astype(np.float32)
     adj = [set() for _ in range(n)]
     for i in range(n):
         for j in range(i + 1, n):

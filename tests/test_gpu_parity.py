@@ -1,0 +1,171 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle and the golden
+fixtures.  Integer work: every comparison is bit-exact."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import complexes as C
+import fixtures as F
+from oracle import oracle as O
+
+import specseqplusplus_amd as S
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "specseqplusplus_amd", "bin", "specseq++")
+
+
+@pytest.fixture(scope="module")
+def h():
+    hd = S.Handle(0)
+    yield hd
+    hd.close()
+
+
+def as_product(mo):
+    return S.Matrix(mo.colptr, mo.rows, mo.dims)
+
+
+def oracle_pairs(mo, model=0):
+    return O.twist_pairs(mo, skip=O.collapse_skip(mo) if model & 2 else None)
+
+
+def check(h, mo, **kw):
+    pairs, ess, st = h.reduce(as_product(mo), S.params(**kw), stats=True)
+    ref = oracle_pairs(mo, kw.get("model", 0))
+    assert pairs.shape == ref.shape and np.array_equal(pairs, ref), kw
+    # every cell is a birth, a death or essential (removed cells excluded)
+    skip = O.collapse_skip(mo) if kw.get("model", 0) & 2 else np.zeros(mo.n, np.uint8)
+    assert 2 * len(pairs) + len(ess) == int((skip == 0).sum())
+    return pairs, ess, st
+
+
+def c1_matrix():
+    X = np.asarray(C.uniform_points(512, 3, 0), dtype=np.float32)
+    m = S.vr_build(X, 0.25, 2)
+    return O.Matrix(m.colptr, m.rows, m.dims)
+
+
+def test_single_triangle_kat_via_cli(tmp_path):
+    for fmt, ext in [("ascii", "dat"), ("binary", "bin")]:
+        out = tmp_path / f"o.{ext}"
+        r = subprocess.run([CLI, f"{F.GOLDEN}/single_triangle.{ext}", str(out), fmt, fmt, "0", "0", "0", "102400",
+                            "1.05", "65536", "1000", "2000"], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        assert out.read_bytes() == open(f"{F.GOLDEN}/single_triangle_persistence_pairs.{ext}", "rb").read()
+
+
+@pytest.mark.parametrize("tri", [C.TORUS_7, C.RP2_6, C.SPHERE_TET, C.BALL_3])
+@pytest.mark.parametrize("seed", [None, 1, 5])
+def test_closed_forms(h, tri, seed):
+    check(h, C.closure_matrix(tri, seed))
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_vr_small(h, seed):
+    X = C.uniform_points(30 + 3 * seed, 3, seed)
+    check(h, C.vr_matrix(X, 0.35 + 0.01 * seed, 2 + seed % 2))
+
+
+@pytest.mark.parametrize("name", F.DATA_FIXTURES)
+def test_shipped_fixtures(h, name):
+    m = S.load_phat(F.fixture_path(name))
+    pairs, _ = h.reduce(m)
+    assert np.array_equal(pairs, F.expected_pairs(name))
+
+
+@pytest.mark.parametrize("kw", [
+    {}, {"level1": 1, "level2": 2}, {"level1": 4, "level2": 8}, {"queue_size": 8}, {"ss_block_size": 64},
+    {"ss_block_size": 1000, "growth": 1.0}, {"ss_block_size": 1 << 30}, {"model": 1}, {"model": 1, "max_iteration": 0},
+    {"model": 2}, {"model": 3}, {"model": 3, "level1": 2, "level2": 3}, {"dualize": 1}])
+def test_config1_variants(h, kw):
+    check(h, c1_matrix(), **kw)
+
+
+def test_guided_clearing_is_exact_and_clears(h):
+    mo = c1_matrix()
+    p0, _, s0 = check(h, mo)
+    p1, _, s1 = check(h, mo, model=1, max_iteration=50)
+    assert np.array_equal(p0, p1) and s1["n_cleared_hd"] > 0 and s1["n_pending"] < s0["n_pending"]
+
+
+def test_model2_uses_reference_kept_edges(h):
+    # the oracle's collapse is pinned to the reference's GUDHI build; the engine's sub-complex
+    # pairs must equal the oracle's pairs of that sub-complex
+    for seed in range(4):
+        X = C.uniform_points(60, 3, 300 + seed)
+        check(h, C.vr_matrix(X, 0.4, 2), model=2)
+
+
+def test_repeat_calls_deterministic(h):
+    mo = c1_matrix()
+    first = h.reduce(as_product(mo))[0]
+    for _ in range(3):
+        assert np.array_equal(h.reduce(as_product(mo))[0], first)
+
+
+@pytest.mark.parametrize("kw", [{"dualize": 1}, {"dualize": 1, "model": 2}, {"dualize": 1, "model": 3},
+                                {"dualize": 1, "level1": 1, "level2": 2, "ss_block_size": 100}])
+def test_dualize_variants_small(h, kw):
+    for seed in range(3):
+        X = C.uniform_points(45, 3, 500 + seed)
+        check(h, C.vr_matrix(X, 0.4, 3), **kw)
+
+
+def test_dualize_device_resident_and_batch(h):
+    mats = []
+    for seed in range(6):
+        X = np.asarray(C.uniform_points(50 + 9 * seed, 3, 60 + seed), dtype=np.float32)
+        mats.append(S.vr_build(X, 0.3, 2))
+    p = S.params(dualize=1)
+    res = h.batch_reduce(mats, p)
+    for m, (pairs, ess) in zip(mats, res):
+        mo = O.Matrix(m.colptr, m.rows, m.dims)
+        ref = O.twist_pairs(mo)
+        assert np.array_equal(pairs, ref)
+        assert 2 * len(pairs) + len(ess) == m.n
+    m = mats[-1]
+    dm = h.upload(m)
+    out = h.device_empty(2 * m.n, np.int64)
+    np_, _ = h.reduce_resident(dm, out, p)
+    assert np.array_equal(out.numpy(2 * np_).reshape(-1, 2), O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))
+
+
+def test_batch_equals_individual(h):
+    mats = []
+    for seed in range(12):
+        X = np.asarray(C.uniform_points(40 + 7 * seed, 3, 40 + seed), dtype=np.float32)
+        mats.append(S.vr_build(X, 0.3, 2))
+    res = h.batch_reduce(mats)
+    for m, (pairs, ess) in zip(mats, res):
+        alone, ess_alone = h.reduce(m)
+        assert np.array_equal(pairs, alone) and np.array_equal(ess, ess_alone)
+        assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))
+
+
+def test_device_resident_entry_point(h):
+    mo = c1_matrix()
+    dm = h.upload(as_product(mo))
+    out = h.device_empty(2 * mo.n, np.int64)
+    np_, st = h.reduce_resident(dm, out)
+    pairs = out.numpy(2 * np_).reshape(-1, 2)
+    assert np.array_equal(pairs, O.twist_pairs(mo))
+
+
+def test_medium_vr_properties(h):
+    X = np.asarray(C.uniform_points(3000, 3, 11), dtype=np.float32)
+    m = S.vr_build(X, 0.1, 2)
+    pairs, ess = h.reduce(m, S.params(model=1))
+    mo = O.Matrix(m.colptr, m.rows, m.dims)
+    assert np.array_equal(pairs, O.twist_pairs(mo))
+    assert (pairs[:, 0] < pairs[:, 1]).all() and (np.diff(pairs[:, 0]) > 0).all()
+    assert (m.dims[pairs[:, 1]] == m.dims[pairs[:, 0]] + 1).all()
+
+
+def test_invalid_parameters_are_rejected(h):
+    mo = c1_matrix()
+    for kw in [{"ss_block_size": 0}, {"growth": 0.5}, {"queue_size": 0}, {"level1": 0}, {"model": 4}]:
+        with pytest.raises(S.SSQError):
+            h.reduce(as_product(mo), S.params(**kw))

@@ -1,0 +1,114 @@
+"""Pins the CPU oracle: KATs, closed forms, independent algorithms, the reference's own
+collapser, and planted mistakes that the checks must catch."""
+import numpy as np
+import pytest
+
+import complexes as C
+import fixtures as F
+from oracle import oracle as O
+
+
+def test_single_triangle_kat_ascii_and_binary(tmp_path):
+    for name, binary in [("single_triangle.dat", False), ("single_triangle.bin", True)]:
+        m = O.load_phat(f"{F.GOLDEN}/{name}", binary)
+        pairs = O.twist_pairs(m)
+        assert pairs.tolist() == [[1, 3], [2, 4], [5, 6]]
+        out = tmp_path / ("p.bin" if binary else "p.dat")
+        O.save_pairs(str(out), pairs, binary)
+        ref = f"{F.GOLDEN}/single_triangle_persistence_pairs." + ("bin" if binary else "dat")
+        assert out.read_bytes() == open(ref, "rb").read()
+
+
+@pytest.mark.parametrize("name,tri,betti", [
+    ("torus", C.TORUS_7, [1, 2, 1]), ("rp2", C.RP2_6, [1, 1, 1]),
+    ("sphere", C.SPHERE_TET, [1, 0, 1]), ("ball", C.BALL_3, [1, 0, 0, 0])])
+@pytest.mark.parametrize("seed", [None, 1, 2, 3])
+def test_closed_form_betti(name, tri, betti, seed):
+    m = C.closure_matrix(tri, seed)
+    p = O.twist_pairs(m)
+    assert C.betti_from_pairs(m, p) == betti
+    assert (p == O.dense_pairs(m)).all()
+    assert (p == O.rank_pairs(m)).all()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_twist_matches_rank_formula_and_standard(seed):
+    X = C.uniform_points(14 + seed % 5, 3, seed)
+    m = C.vr_matrix(X, 0.6, 3)
+    if m.n > 128:
+        m = C.vr_matrix(X, 0.45, 2)
+    p = O.twist_pairs(m)
+    assert np.array_equal(p, O.dense_pairs(m))
+    if m.n <= 128:
+        assert np.array_equal(p, O.rank_pairs(m))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_twist_matches_standard_medium(seed):
+    X = C.uniform_points(60, 3, 50 + seed)
+    m = C.vr_matrix(X, 0.35, 2)
+    assert np.array_equal(O.twist_pairs(m), O.dense_pairs(m))
+
+
+def test_pair_invariants_on_fixtures():
+    for name in F.DATA_FIXTURES:
+        path = F.fixture_path(name)
+        m = O.load_phat(path)
+        p = O.twist_pairs(m)
+        assert np.array_equal(p, F.expected_pairs(name))
+        assert (p[:, 0] < p[:, 1]).all()
+        assert (np.diff(p[:, 0]) > 0).all()
+        assert (m.dims[p[:, 1]] == m.dims[p[:, 0]] + 1).all()
+        assert len(np.unique(p)) == 2 * len(p)
+
+
+def test_collapse_restatement_matches_reference_fixtures():
+    for u, v, kept in F.kept_edge_cases():
+        assert np.array_equal(O.collapse_edges(u, v), kept)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref (reference collapser) not built here")
+@pytest.mark.parametrize("seed", range(10))
+def test_collapse_restatement_matches_reference_build(seed):
+    X = C.uniform_points(40 + 10 * seed, 3, 900 + seed)
+    m = C.vr_matrix(X, 0.3 + 0.03 * seed, 1)
+    e = np.where(m.dims == 1)[0]
+    u, v = m.rows[m.colptr[e]], m.rows[m.colptr[e] + 1]
+    assert np.array_equal(O.collapse_edges(u, v), O.ref_collapse_edges(u, v))
+
+
+def test_collapse_subcomplex_invariants():
+    X = C.uniform_points(50, 3, 7)
+    m = C.vr_matrix(X, 0.45, 2)
+    skip = O.collapse_skip(m)
+    e = np.where(m.dims == 1)[0]
+    assert not skip[m.dims == 0].any()
+    assert skip[e[0]] == 0  # the first edge is always kept
+    # removed cells never appear as faces of kept cells
+    for c in np.where(skip == 0)[0]:
+        assert not skip[m.rows[m.colptr[c]:m.colptr[c + 1]]].any()
+    # collapse preserves H0 of the whole flag complex
+    p_full = O.twist_pairs(m)
+    p_sub = O.twist_pairs(m, skip=skip)
+    ess_full = C.betti_from_pairs(m, p_full)[0]
+    paired = set(p_sub.ravel().tolist())
+    ess_sub0 = sum(1 for c in np.where(m.dims == 0)[0] if c not in paired)
+    assert ess_full == ess_sub0
+
+
+def test_planted_mistakes_are_caught():
+    """A wrong pairing must fail the independent checks used above."""
+    m = C.closure_matrix(C.TORUS_7, 2)
+    good = O.rank_pairs(m)
+    bad = good.copy()
+    bad[0, 1] += 1  # wrong death index
+    assert not np.array_equal(bad, O.twist_pairs(m))
+    # dropping a boundary entry changes the pairing or the Betti numbers
+    rows = m.rows.copy()
+    rows[-1] = rows[-2]
+    cols = [sorted(set(rows[m.colptr[c]:m.colptr[c + 1]].tolist())) for c in range(m.n)]
+    m2 = O.Matrix.from_columns(cols, m.dims)
+    assert C.betti_from_pairs(m2, O.twist_pairs(m2)) != [1, 2, 1]
+    # a collapse that keeps every edge differs from GUDHI's kept set on a dense graph
+    u, v, kept = F.kept_edge_cases()[1]
+    assert not np.array_equal(np.ones_like(kept), kept)

@@ -258,6 +258,9 @@ def main():
         "served": served, "parity": parity,
         "engine": {"pending": st["n_pending"], "apparent": st["n_apparent"], "cleared_hd": st["n_cleared_hd"],
                    "additions": st["n_additions"], "rounds": st["n_rounds"], "pages": st["n_pages"],
+                   "tiled_additions": st["n_tiled_additions"],
+                   "max_chain": max(x["max_chain"] for x in stats[-len(plans):]),
+                   "max_addition_len": max(x["max_addition_len"] for x in stats[-len(plans):]),
                    "t_init_ms": round(st["t_init_ms"], 3), "t_reduce_ms": round(st["t_reduce_ms"], 3)},
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -72,11 +72,18 @@ typedef struct ssq_stats {
   int64_t n_pages;       /* pages run                                                  */
   int64_t n_rounds;      /* claim/add rounds run                                       */
   int64_t n_additions;   /* column additions performed                                 */
-  double add_bytes;      /* algorithmic bytes of those additions: 4(|a|+|b|+|a^b|)+24 */
+  double add_bytes;      /* algorithmic bytes of the short additions (LDS merges):
+                            4(|a|+|b|+|a^b|)+24 each                                   */
   int64_t store_bytes;   /* device bytes reserved for the column store                 */
   double t_add_kernel_ms; /* summed duration of the column-addition kernel launches, from
                              HIP events on the engine stream (SSQ_KERNEL_TIMING=1), else 0 */
   int64_t n_add_launches; /* column-addition kernel launches                            */
+  int64_t n_tiled_additions; /* additions merged tile by tile from HBM                   */
+  int64_t n_dense_additions; /* additions into a workgroup's LDS bit-set accumulator      */
+  double dense_bytes;     /* algorithmic bytes of the long-column kernel's additions     */
+  double t_add_long_kernel_ms; /* summed duration of the long-column kernel launches     */
+  int64_t max_chain;      /* most additions one column made in one round                */
+  int64_t max_addition_len; /* largest |a|+|b| of one addition                          */
 } ssq_stats;
 
 /* ---- I/O (main.cu:90-167 processFiles; main.cu:191-229 writeVectorToFile) ---------- */

@@ -124,13 +124,13 @@ def _skip(m, skip):
 def twist_pairs(m, skip=None, stats=False):
     """Persistence pairs (np x 2 int64, sorted by birth) by twist reduction."""
     out = np.zeros((max(m.n, 1), 2), dtype=np.int64)
-    st = np.zeros(2, dtype=np.float64)
+    st = np.zeros(3, dtype=np.float64)
     s, sp = _skip(m, skip)
     k = lib().or_twist_reduce(m.n, _p(m.colptr, _i64p), _p(m.rows, _i32p), _p(m.dims, _i32p),
                               sp, _p(out, _i64p), _p(st, _dp))
     pairs = out[:k].copy()
     if stats:
-        return pairs, {"additions": float(st[0]), "bytes": float(st[1])}
+        return pairs, {"additions": float(st[0]), "bytes": float(st[1]), "max_chain": float(st[2])}
     return pairs
 
 
@@ -166,6 +166,23 @@ def ref_collapse_edges(u, v):
     kept = np.zeros(len(u), dtype=np.uint8)
     ref().ref_collapse_edges(len(u), _p(u, _i32p), _p(v, _i32p), _p(kept, _u8p))
     return kept
+
+
+def dual_matrix(m):
+    """Anti-transpose (cohomology): cell c -> n-1-c, dimension maxdim - dim."""
+    n = m.n
+    cols = [[] for _ in range(n)]
+    for c in range(n - 1, -1, -1):
+        for r in m.rows[m.colptr[c]:m.colptr[c + 1]]:
+            cols[n - 1 - int(r)].append(n - 1 - c)
+    maxd = int(m.dims.max()) if n else 0
+    dims = (maxd - m.dims)[::-1]
+    return Matrix.from_columns(cols, dims)
+
+
+def dual_pairs_to_primal(pairs, n):
+    out = np.stack([n - 1 - pairs[:, 1], n - 1 - pairs[:, 0]], 1) if len(pairs) else pairs
+    return out[np.argsort(out[:, 0])] if len(out) else out
 
 
 def collapse_skip(m):

@@ -260,7 +260,7 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
   int64_t acap = 1 << 20, aused = 0;
   int32_t* arena = (int32_t*)malloc(sizeof(int32_t) * (size_t)acap);
   int32_t* ext = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
-  double adds = 0, bytes = 0;
+  double adds = 0, bytes = 0, max_chain = 0;
   bitree t;
   bt_init(&t, n);
   for (int32_t d = maxd; d >= 1; d--) {
@@ -268,7 +268,9 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
       if (dims[j] != d || cleared[j] || (skip && skip[j])) continue;
       for (int64_t k = colptr[j]; k < colptr[j + 1]; k++) bt_toggle(&t, rows[k]);
       int64_t low;
+      double col_adds = 0;
       while ((low = bt_max(&t)) >= 0 && pivot_of[low] >= 0) {
+        col_adds += 1;
         int32_t p = pivot_of[low];
         int64_t a = t.count;
         const int32_t* col = arena + roff[p];
@@ -276,6 +278,7 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
         adds += 1;
         bytes += 4.0 * (double)(a + rlen[p] + t.count) + 24.0;
       }
+      if (col_adds > max_chain) max_chain = col_adds;
       if (low < 0) continue;
       /* store the reduced column (descending extraction, then reverse) */
       int64_t len = 0;
@@ -303,6 +306,7 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
   if (stats_out) {
     stats_out[0] = adds;
     stats_out[1] = bytes;
+    stats_out[2] = max_chain;
   }
   free(dob);
   free(pivot_of);

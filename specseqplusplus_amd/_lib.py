@@ -35,7 +35,10 @@ class ssq_stats(ctypes.Structure):
                 ("n_cleared_hd", ctypes.c_int64), ("n_kept_edges", ctypes.c_int64), ("n_pages", ctypes.c_int64),
                 ("n_rounds", ctypes.c_int64), ("n_additions", ctypes.c_int64), ("add_bytes", ctypes.c_double),
                 ("store_bytes", ctypes.c_int64), ("t_add_kernel_ms", ctypes.c_double),
-                ("n_add_launches", ctypes.c_int64)]
+                ("n_add_launches", ctypes.c_int64), ("n_tiled_additions", ctypes.c_int64),
+                ("n_dense_additions", ctypes.c_int64), ("dense_bytes", ctypes.c_double),
+                ("t_add_long_kernel_ms", ctypes.c_double),
+                ("max_chain", ctypes.c_int64), ("max_addition_len", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

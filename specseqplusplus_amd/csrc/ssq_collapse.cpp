@@ -51,21 +51,35 @@ int64_t common(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* out, 
   return k;
 }
 
+// does w's closed valid neighbourhood contain all of I[0:m)?
 template <typename Valid>
-bool dominated(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* I) {
-  int64_t m = common(g, a, b, valid, I, nullptr, nullptr);
+inline bool covers(const Graph& g, int32_t w, const int32_t* I, int64_t m, Valid valid) {
+  int64_t p = g.off[w], pe = g.off[w + 1];
+  for (int64_t t = 0; t < m; t++) {
+    const int32_t x = I[t];
+    if (x == w) continue;
+    while (p < pe && g.nbr[p] < x) p++;
+    if (!(p < pe && g.nbr[p] == x && valid(g.eid[p]))) return false;
+  }
+  return true;
+}
+
+// The predicate of edge_is_dominated: |I| == 1 or some w in I covers I.  *witness is tried
+// first and receives the dominating vertex found (the answer does not depend on the order
+// in which candidates are tried).
+template <typename Valid>
+bool dominated(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* I, int32_t* witness) {
+  const int64_t m = common(g, a, b, valid, I, nullptr, nullptr);
   if (m <= 1) return m == 1;
+  const int32_t hint = *witness;
+  if (hint >= 0 && std::binary_search(I, I + m, hint) && covers(g, hint, I, m, valid)) return true;
   for (int64_t c = 0; c < m; c++) {
-    int32_t w = I[c];
-    int64_t p = g.off[w], pe = g.off[w + 1];
-    bool ok = true;
-    for (int64_t t = 0; t < m && ok; t++) {
-      int32_t x = I[t];
-      if (x == w) continue;
-      while (p < pe && g.nbr[p] < x) p++;
-      ok = p < pe && g.nbr[p] == x && valid(g.eid[p]);
+    const int32_t w = I[c];
+    if (w == hint) continue;
+    if (covers(g, w, I, m, valid)) {
+      *witness = w;
+      return true;
     }
-    if (ok) return true;
   }
   return false;
 }
@@ -113,6 +127,7 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
 
   // 1. main-loop decisions, in parallel
   std::vector<uint8_t> main_crit((size_t)ne, 0);
+  std::vector<int32_t> witness((size_t)ne, -1);
   main_crit[0] = 1;
   if (threads < 1) threads = 1;
   {
@@ -125,7 +140,7 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
         int64_t e = std::min(ne, s + 256);
         for (int64_t i = s; i < e; i++) {
           int32_t idx = (int32_t)i;
-          main_crit[i] = !dominated(g, eu[i], ev[i], [idx](int32_t x) { return x < idx; }, I.data());
+          main_crit[i] = !dominated(g, eu[i], ev[i], [idx](int32_t x) { return x < idx; }, I.data(), &witness[i]);
         }
       }
     };
@@ -163,7 +178,8 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
       heap.pop();
       if (g.crit[cur]) continue;
       const uint8_t* crit = g.crit.data();
-      if (dominated(g, eu[cur], ev[cur], [cur, crit](int32_t x) { return x < cur || crit[x]; }, I.data()))
+      if (dominated(g, eu[cur], ev[cur], [cur, crit](int32_t x) { return x < cur || crit[x]; }, I.data(),
+                    &witness[cur]))
         continue;
       g.crit[cur] = 1;
       kept[cur] = 1;

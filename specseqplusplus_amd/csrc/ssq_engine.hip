@@ -58,16 +58,36 @@ struct Dev {
   uint8_t* flag;       // scratch per-cell flag (clear / negative)
   uint32_t* piv;       // row -> column
   uint32_t* leftone;   // row -> leftmost column containing it
-  int32_t* work[4];    // W0, W1, A0, A1
+  int32_t* work[5];    // W0, W1, A0, A1, long list
   uint32_t* ctr;       // [0..1] cW, [2..3] cA, [4] apparent, [5] hd-cleared
   unsigned long long* arena_used;  // ints used past arena_base
   int64_t arena_base, arena_cap;
   uint32_t* lvl_cnt;   // slots taken per level 1..3
   int32_t lvl_cap[4];
   uint32_t lvl_slots[4];
-  unsigned long long* stats;  // [0] additions, [1] algorithmic bytes
+  unsigned long long* stats;  // counters at cw::kStats
   uint32_t* err;
+  int32_t short_max;    // largest |a|+|b| merged in a wave's LDS (<= kTile)
+  int32_t long_window;  // rows covered by the long-column bit set (<= kLongBits)
 };
+
+// Word offsets in the per-call counter block (64 x uint32 in device memory).
+namespace cw {
+constexpr int kW = 0;          // [0..1] worklist counts (claim input), double-buffered by round
+constexpr int kA = 2;          // [2..3] add-list counts
+constexpr int kApparent = 4;   // apparent pairs found by the init passes
+constexpr int kHdCleared = 5;  // cells cleared by guided clearing
+constexpr int kLong = 6;       // long-column list count (this round)
+constexpr int kFlags = 8;      // [8..9] input validation flags
+constexpr int kRowStat = 16;   // [16..23] max low+1 per dim, [24..31] min first row per dim
+constexpr int kStats = 32;     // 8 x u64: additions, bytes, tiled additions, dense additions, dense bytes
+constexpr int kArena = 48;     // u64: arena entries used
+constexpr int kLvl = 52;       // [52..55] level slot counters
+constexpr int kErr = 56;       // store exhausted
+constexpr int kMaxDim = 58;    // [58] max dimension, [59] bad dimension flag
+constexpr int kChain = 60;     // [60] longest chain, [61] largest |a|+|b|
+constexpr int kWords = 64;
+}  // namespace cw
 
 #define GRID_STRIDE(i, n) \
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
@@ -158,7 +178,7 @@ __global__ void k_lookup(Dev d, const uint8_t* hdclear) {
     if (d.leftone[lw] == (uint32_t)c) {
       d.piv[lw] = (uint32_t)c;
       d.state[c] = ST_PIVOT;
-      atomicAdd(&d.ctr[4], 1u);
+      atomicAdd(&d.ctr[cw::kApparent], 1u);
     }
     const int32_t* col = d.pool + d.off[c];
     for (int32_t k = 0; k < l; k++)
@@ -198,22 +218,59 @@ __global__ void k_compress(Dev d) {
   }
 }
 
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+  for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+  for (int d = kWave / 2; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+// one atomic per wavefront (a plain per-thread atomic on one address serialises millions)
 __global__ void k_maxdim(int64_t n, const int32_t* dims, int32_t* out, uint32_t* bad) {
-  GRID_STRIDE(c, n) {
-    const int32_t x = dims[c];
-    if (x < 0 || x >= kMaxDims) atomicOr(bad, 1u);
-    else atomicMax(out, x);
+  int32_t mx = -1, b = 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = base + threadIdx.x;
+    if (c < n) {
+      const int32_t x = dims[c];
+      if (x < 0 || x >= kMaxDims) b = 1;
+      else mx = max(mx, x);
+    }
+  }
+  mx = wave_max(mx);
+  b = wave_max(b);
+  if (lane_id() == 0) {
+    if (mx >= 0) atomicMax(out, mx);
+    if (b) atomicOr(bad, 1u);
   }
 }
 
 // Per-dimension page bounds: max low over pending columns (+1) and min first row over
 // non-zero columns (specseq.h:365-409).
 __global__ void k_dimstats(Dev d, int32_t* maxrow, int32_t* minrow) {
-  GRID_STRIDE(c, d.n) {
+  int32_t mx[kMaxDims], mn[kMaxDims];
+  for (int k = 0; k < kMaxDims; k++) {
+    mx[k] = -1;
+    mn[k] = 0x7fffffff;
+  }
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < d.n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = base + threadIdx.x;
+    if (c >= d.n) continue;
     const int32_t dm = d.dims[c];
     if (dm < 0 || dm >= kMaxDims || d.len[c] == 0) continue;
-    if (d.state[c] == ST_PENDING) atomicMax(&maxrow[dm], d.low[c] + 1);
-    atomicMin(&minrow[dm], d.pool[d.off[c]]);
+    for (int k = 0; k < kMaxDims; k++)
+      if (k == dm) {  // static register indexing
+        if (d.state[c] == ST_PENDING) mx[k] = max(mx[k], d.low[c] + 1);
+        mn[k] = min(mn[k], d.pool[d.off[c]]);
+      }
+  }
+  for (int k = 0; k < kMaxDims; k++) {
+    const int32_t a = wave_max(mx[k]), b = wave_min(mn[k]);
+    if (lane_id() == 0) {
+      if (a >= 0) atomicMax(&maxrow[k], a);
+      if (b != 0x7fffffff) atomicMin(&minrow[k], b);
+    }
   }
 }
 
@@ -311,7 +368,7 @@ __global__ void k_scan_window(Dev d, const int32_t* cols, int32_t ncols, int32_t
       c = cols[i];
       act = d.state[c] == ST_PENDING && d.low[c] >= minrow;
     }
-    wave_push(act, c, d.work[0], &d.ctr[0]);
+    wave_push(act, c, d.work[0], &d.ctr[cw::kW]);
   }
 }
 
@@ -319,10 +376,11 @@ __global__ void k_scan_window(Dev d, const int32_t* cols, int32_t ncols, int32_t
 __global__ void k_claim(Dev d, int r) {
   const int in = r & 1, nx = (r + 1) & 1;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    d.ctr[2 + nx] = 0;
-    d.ctr[nx] = 0;
+    d.ctr[cw::kA + nx] = 0;
+    d.ctr[cw::kW + nx] = 0;
+    d.ctr[cw::kLong] = 0;  // the long list of the previous round is consumed
   }
-  const uint32_t cnt = d.ctr[in];
+  const uint32_t cnt = d.ctr[cw::kW + in];
   const int32_t* list = d.work[in];
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < cnt; base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = base + threadIdx.x;
@@ -342,8 +400,33 @@ __global__ void k_claim(Dev d, int r) {
         who = c;
       }
     }
-    wave_push(add, who, d.work[2 + in], &d.ctr[2 + in]);
+    wave_push(add, who, d.work[2 + in], &d.ctr[cw::kA + in]);
   }
+}
+
+
+// One wavefront: O = A xor B for sorted sets in HBM, merged kTile merged positions at a
+// time through the wave's LDS staging buffers.  Returns |O|.  O must hold |A|+|B|.
+__device__ int wave_tiled_xor(const int32_t* A, int32_t a_len, const int32_t* B, int32_t b_len, int32_t* O,
+                              int32_t* bufA, int32_t* bufB, int32_t* bufO) {
+  int32_t ia = 0, jb = 0, written = 0;
+  while (ia < a_len || jb < b_len) {
+    const int32_t ca = min(kTile, a_len - ia), cb = min(kTile, b_len - jb);
+    wave_copy(bufA, A + ia, ca);
+    wave_copy(bufB, B + jb, cb);
+    wave_sync();
+    const int32_t K = min(kTile, ca + cb);
+    int32_t jE;
+    const int32_t iE = split_adj(bufA, ca, bufB, cb, K, &jE);
+    const int32_t m = wave_xor_lds(bufA, iE, bufB, jE, bufO);
+    wave_copy(O + written, bufO, m);
+    wave_sync();
+    written += m;
+    ia += iE;
+    jb += jE;
+  }
+  __threadfence();
+  return written;
 }
 
 // Add round r: one wavefront per listed column; it adds piv[low] while that pivot is a
@@ -355,7 +438,7 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
   const int in = r & 1, nx = (r + 1) & 1;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
-  const uint32_t cnt = d.ctr[2 + in];
+  const uint32_t cnt = d.ctr[cw::kA + in];
   const int32_t* list = d.work[2 + in];
   const int64_t gw = (int64_t)blockIdx.x * kAddWaves + w;
   const int64_t nw = (int64_t)gridDim.x * kAddWaves;
@@ -363,6 +446,7 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
   int32_t* bufB = lds[w][1];
   int32_t* bufO = lds[w][2];
   unsigned long long n_add = 0, n_bytes = 0;
+  uint32_t max_chain = 0, max_len = 0;
 
   for (int64_t it = gw; it < cnt; it += nw) {
     const int32_t c = list[it];
@@ -374,13 +458,15 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
     bool failed = false;
     int32_t* sA = bufA;
     int32_t* sO = bufO;
+    uint32_t chain = 0;
+    bool deferred = false;
     for (;;) {
       const uint32_t p = d.piv[l];
       if (p == kEmpty || p >= (uint32_t)c) break;
       const int32_t b_len = d.len[p];
       const int32_t* B = d.pool + d.off[p];
       int32_t r_len;
-      if (a_len + b_len <= kTile) {
+      if (a_len + b_len <= d.short_max) {
         if (!in_lds) {
           wave_copy(sA, d.pool + a_off, a_len);
           in_lds = true;
@@ -392,40 +478,20 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
         sA = sO;
         sO = t;
       } else {
-        if (in_lds) {  // the running column goes back to the store first
+        // too long for this wave's LDS: the column moves to the long list, where a whole
+        // workgroup continues the chase in the same add round
+        if (in_lds) {
           if (a_len > a_cap && !wave_alloc(d, a_len, &a_off, &a_cap)) { failed = true; break; }
           wave_copy(d.pool + a_off, sA, a_len);
           in_lds = false;
           __threadfence();
         }
-        int64_t o_off;
-        int32_t o_cap;
-        if (!wave_alloc(d, a_len + b_len, &o_off, &o_cap)) { failed = true; break; }
-        // tiled merge path: chunks of A and B staged in LDS; kTile merged positions per tile
-        const int32_t* A = d.pool + a_off;
-        int32_t* O = d.pool + o_off;
-        int32_t ia = 0, jb = 0, written = 0;
-        while (ia < a_len || jb < b_len) {
-          const int32_t ca = min(kTile, a_len - ia), cb = min(kTile, b_len - jb);
-          wave_copy(bufA, A + ia, ca);
-          wave_copy(bufB, B + jb, cb);
-          wave_sync();
-          const int32_t K = min(kTile, ca + cb);
-          int32_t jE;
-          const int32_t iE = split_adj(bufA, ca, bufB, cb, K, &jE);
-          const int32_t m = wave_xor_lds(bufA, iE, bufB, jE, bufO);
-          wave_copy(O + written, bufO, m);
-          wave_sync();
-          written += m;
-          ia += iE;
-          jb += jE;
-        }
-        a_off = o_off;
-        a_cap = o_cap;
-        r_len = written;
-        __threadfence();
+        deferred = true;
+        break;
       }
       n_add += 1;
+      chain += 1;
+      max_len = max(max_len, (uint32_t)(a_len + b_len));
       n_bytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
       a_len = r_len;
       if (a_len == 0) {
@@ -435,6 +501,7 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
       l = in_lds ? sA[a_len - 1] : d.pool[a_off + a_len - 1];
       if (l < minrow) break;
     }
+    max_chain = max(max_chain, chain);
     if (!failed && in_lds) {
       if (a_len > a_cap && !wave_alloc(d, a_len, &a_off, &a_cap)) failed = true;
       else wave_copy(d.pool + a_off, sA, a_len);
@@ -445,6 +512,19 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
         d.low[c] = -1;
         d.state[c] = ST_ZERO;
       }
+      continue;
+    }
+    if (deferred) {
+      if (lane == 0) {
+        d.off[c] = a_off;
+        d.cap[c] = a_cap;
+        d.len[c] = a_len;
+        d.low[c] = l;
+        d.state[c] = ST_PENDING;
+        const uint32_t pos = atomicAdd(&d.ctr[cw::kLong], 1u);
+        d.work[4][pos] = c;
+      }
+      wave_sync();
       continue;
     }
     bool again = false;
@@ -460,7 +540,7 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
       }
     }
     if (lane == 0 && again) {
-      const uint32_t pos = atomicAdd(&d.ctr[nx], 1u);
+      const uint32_t pos = atomicAdd(&d.ctr[cw::kW + nx], 1u);
       d.work[nx][pos] = c;
     }
     wave_sync();
@@ -468,6 +548,246 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
   if (lane == 0 && n_add) {
     atomicAdd(&d.stats[0], n_add);
     atomicAdd(&d.stats[1], n_bytes);
+    atomicMax(&d.ctr[cw::kChain], max_chain);
+    atomicMax(&d.ctr[cw::kChain + 1], max_len);
+  }
+}
+
+
+// Long columns of an add round: one workgroup per column.  While the column's rows lie in a
+// window of kLongBits rows below its low, it is kept as a bit set in LDS: adding a pivot
+// toggles the pivot's rows (LDS atomics) and the new low is the highest set bit, found by a
+// block-wide scan downwards from the old low.  This reads only the pivot per addition
+// (a sorted merge reads and writes the whole running column).  A pivot with rows below the
+// window sends the column back to sorted form and tiled merges.
+constexpr int kLongBits = 1 << 19;
+constexpr int kLongWords = kLongBits / 32;
+constexpr int kLongThreads = 256;
+
+__device__ __forceinline__ int32_t block_max_i32(int32_t v, int32_t* scratch) {
+  v = wave_max(v);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane_id() == 0) scratch[w] = v;
+  __syncthreads();
+  int32_t m = scratch[0];
+  for (int k = 1; k < kLongThreads / kWave; k++) m = max(m, scratch[k]);
+  return m;
+}
+
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* scratch, int32_t* total) {
+  const int w = threadIdx.x / kWave;
+  const int incl = wave_incl_scan(v);
+  __syncthreads();
+  if (lane_id() == kWave - 1) scratch[w] = incl;
+  __syncthreads();
+  int32_t base = 0, tot = 0;
+  for (int k = 0; k < kLongThreads / kWave; k++) {
+    if (k < w) base += scratch[k];
+    tot += scratch[k];
+  }
+  *total = tot;
+  return base + incl - v;
+}
+
+__global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t minrow) {
+  __shared__ uint32_t bits[kLongWords];
+  __shared__ int32_t stage[3][kTileBuf];
+  __shared__ int32_t red[kLongThreads / kWave];
+  __shared__ int32_t sh_count, sh_flag;
+  __shared__ int64_t sh_off;
+  __shared__ int32_t sh_cap;
+  const int nx = (r + 1) & 1;
+  const int tid = threadIdx.x;
+  const uint32_t cnt = d.ctr[cw::kLong];
+  const int32_t* list = d.work[4];
+  unsigned long long n_add = 0, n_dense = 0, n_dbytes = 0, n_tiled = 0;
+  uint32_t max_chain = 0, max_len = 0;
+
+  for (int64_t it = blockIdx.x; it < cnt; it += gridDim.x) {
+    const int32_t c = list[it];
+    int64_t a_off = d.off[c];
+    int32_t a_cap = d.cap[c];
+    int32_t a_len = d.len[c];
+    int32_t l = d.low[c];
+    const int32_t lo = l - d.long_window + 1;  // window [lo, l] of the bit set
+    bool dense = d.pool[a_off] >= lo;
+    bool failed = false;
+    uint32_t chain = 0;
+    if (dense) {
+      const int32_t nw = ((l - lo) >> 5) + 1;
+      for (int32_t w = tid; w < nw; w += kLongThreads) bits[w] = 0u;
+      __syncthreads();
+      const int32_t* A = d.pool + a_off;
+      for (int32_t k = tid; k < a_len; k += kLongThreads) {
+        const int32_t x = A[k] - lo;
+        atomicOr(&bits[x >> 5], 1u << (x & 31));
+      }
+      if (tid == 0) sh_count = a_len;
+      __syncthreads();
+    }
+    for (;;) {
+      const uint32_t p = d.piv[l];
+      if (p == kEmpty || p >= (uint32_t)c) break;
+      const int32_t b_len = d.len[p];
+      const int32_t* B = d.pool + d.off[p];
+      if (dense && B[0] < lo) {
+        // leave the bit set: write the column back in sorted form
+        int32_t W = ((l - lo) >> 5) + 1, total;
+        const int32_t per = (W + kLongThreads - 1) / kLongThreads;
+        const int32_t w0 = min(W, tid * per), w1 = min(W, w0 + per);
+        int32_t pc = 0;
+        for (int32_t w = w0; w < w1; w++) pc += __popc(bits[w]);
+        const int32_t pos0 = block_excl_scan(pc, red, &total);
+        if (tid == 0) {
+          int64_t o = a_off;
+          int32_t cp = a_cap;
+          sh_flag = 1;
+          if (total > cp && !store_alloc(d, total, &o, &cp)) sh_flag = 0;
+          sh_off = o;
+          sh_cap = cp;
+        }
+        __syncthreads();
+        if (!sh_flag) { failed = true; break; }
+        a_off = sh_off;
+        a_cap = sh_cap;
+        int32_t* O = d.pool + a_off;
+        int32_t pos = pos0;
+        for (int32_t w = w0; w < w1; w++) {
+          uint32_t x = bits[w];
+          while (x) {
+            const int b = __ffs(x) - 1;
+            O[pos++] = lo + (w << 5) + b;
+            x &= x - 1;
+          }
+        }
+        __threadfence();
+        __syncthreads();
+        a_len = total;
+        dense = false;
+      }
+      int32_t r_len;
+      if (dense) {
+        // toggle the pivot's rows
+        int32_t delta = 0;
+        for (int32_t k = tid; k < b_len; k += kLongThreads) {
+          const int32_t x = B[k] - lo;
+          const uint32_t m = 1u << (x & 31);
+          const uint32_t old = atomicXor(&bits[x >> 5], m);
+          delta += (old & m) ? -1 : 1;
+        }
+        delta = wave_incl_scan(delta);
+        if (lane_id() == kWave - 1) atomicAdd(&sh_count, delta);
+        __syncthreads();
+        r_len = sh_count;
+        // new low: highest set bit, scanning down from the old low's word
+        int32_t found = -1;
+        for (int32_t base = (l - lo) >> 5; base >= 0 && found < 0; base -= kLongThreads) {
+          const int32_t w = base - tid;
+          const int32_t cand = (w >= 0 && bits[w]) ? w : -1;
+          found = block_max_i32(cand, red);
+        }
+        const int32_t newl = found < 0 ? -1 : lo + (found << 5) + (31 - __clz(bits[found]));
+        n_dense += 1;
+        n_dbytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
+        max_len = max(max_len, (uint32_t)(a_len + b_len));
+        chain += 1;
+        a_len = r_len;
+        l = r_len ? newl : -1;
+        __syncthreads();
+      } else {
+        // tiled merge by wave 0 into a fresh slot
+        if (tid == 0) {
+          int64_t o;
+          int32_t cp;
+          sh_flag = store_alloc(d, a_len + b_len, &o, &cp) ? 1 : 0;
+          sh_off = o;
+          sh_cap = cp;
+        }
+        __syncthreads();
+        if (!sh_flag) { failed = true; break; }
+        if (tid < kWave)
+          r_len = wave_tiled_xor(d.pool + a_off, a_len, B, b_len, d.pool + sh_off, stage[0], stage[1], stage[2]);
+        if (tid == 0) sh_count = r_len;
+        __syncthreads();
+        r_len = sh_count;
+        n_tiled += 1;
+        n_add += 1;
+        n_dbytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
+        max_len = max(max_len, (uint32_t)(a_len + b_len));
+        chain += 1;
+        a_off = sh_off;
+        a_cap = sh_cap;
+        a_len = r_len;
+        l = r_len ? d.pool[a_off + r_len - 1] : -1;
+        __syncthreads();
+      }
+      if (l < 0 || l < minrow) break;
+    }
+    max_chain = max(max_chain, chain);
+    if (!failed && dense) {  // write the bit set back in sorted form
+      const int32_t W = l >= 0 ? ((l - lo) >> 5) + 1 : 0;
+      int32_t total;
+      const int32_t per = (W + kLongThreads - 1) / kLongThreads;
+      const int32_t w0 = min(W, tid * per), w1 = min(W, w0 + per);
+      int32_t pc = 0;
+      for (int32_t w = w0; w < w1; w++) pc += __popc(bits[w]);
+      const int32_t pos0 = block_excl_scan(pc, red, &total);
+      if (tid == 0) {
+        int64_t o = a_off;
+        int32_t cp = a_cap;
+        sh_flag = 1;
+        if (total > cp && !store_alloc(d, total, &o, &cp)) sh_flag = 0;
+        sh_off = o;
+        sh_cap = cp;
+      }
+      __syncthreads();
+      if (!sh_flag) failed = true;
+      else {
+        a_off = sh_off;
+        a_cap = sh_cap;
+        int32_t* O = d.pool + a_off;
+        int32_t pos = pos0;
+        for (int32_t w = w0; w < w1; w++) {
+          uint32_t x = bits[w];
+          while (x) {
+            const int b = __ffs(x) - 1;
+            O[pos++] = lo + (w << 5) + b;
+            x &= x - 1;
+          }
+        }
+        a_len = total;
+      }
+    }
+    if (tid == 0) {
+      if (failed) {
+        d.len[c] = 0;
+        d.low[c] = -1;
+        d.state[c] = ST_ZERO;
+      } else {
+        d.off[c] = a_off;
+        d.cap[c] = a_cap;
+        d.len[c] = a_len;
+        d.low[c] = a_len ? l : -1;
+        d.state[c] = a_len ? ST_PENDING : ST_ZERO;
+        if (a_len && l >= minrow) {
+          const uint32_t p = d.piv[l];
+          if (p == kEmpty || p > (uint32_t)c) {
+            const uint32_t pos = atomicAdd(&d.ctr[cw::kW + nx], 1u);
+            d.work[nx][pos] = c;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && (n_add || n_dense)) {
+    atomicAdd(&d.stats[0], n_add + n_dense);
+    atomicAdd(&d.stats[2], n_tiled);
+    atomicAdd(&d.stats[3], n_dense);
+    atomicAdd(&d.stats[4], n_dbytes);
+    atomicMax(&d.ctr[cw::kChain], max_chain);
+    atomicMax(&d.ctr[cw::kChain + 1], max_len);
   }
 }
 
@@ -937,12 +1257,16 @@ class Engine {
     }
     HIPCHK(hipSetDevice(device_));
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    HIPCHK(hipHostMalloc(&host_ctr_, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&host_ctr_, cw::kWords * sizeof(uint32_t), hipHostMallocDefault));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_));
     cus_ = prop.multiProcessorCount;
     const char* kt = std::getenv("SSQ_KERNEL_TIMING");
     timing_ = kt && kt[0] == '1';
+    // test hooks: smaller limits push short columns through the long-column paths
+    if (const char* e = std::getenv("SSQ_TEST_SHORT_MAX")) short_max_ = std::max(2, std::min(kTile, std::atoi(e)));
+    if (const char* e = std::getenv("SSQ_TEST_WINDOW_BITS"))
+      long_window_ = std::max(64, std::min(kLongBits, std::atoi(e)));
     if (timing_)
       for (auto& e : ev_) HIPCHK(hipEventCreate(&e));
     return SSQ_OK;
@@ -1016,6 +1340,8 @@ class Engine {
       b_drem_, b_scan_;
   int64_t arena_hint_ = 0;
   bool timing_ = false;
+  int32_t short_max_ = kTile;
+  int32_t long_window_ = kLongBits;
   hipEvent_t ev_[8] = {};
 };
 
@@ -1119,14 +1445,14 @@ int Engine::collapse(Dev& d, int64_t n, const int32_t* d_dims, const int64_t* d_
   if (ne == 0) return SSQ_OK;
   int32_t* eu = ecols + n;
   int32_t* ev = ecols + 2 * n;
-  HIPCHK(hipMemsetAsync(d.ctr + 8, 0, sizeof(uint32_t), stream_));
+  HIPCHK(hipMemsetAsync(d.ctr + cw::kFlags, 0, sizeof(uint32_t), stream_));
   hipLaunchKernelGGL(k_edge_ends, dim3(grid_for(ne)), dim3(256), 0, stream_, ne, ecols, d_colptr, d_rows, eu, ev,
-                     d.ctr + 8);
+                     d.ctr + cw::kFlags);
   std::vector<int32_t> hu((size_t)ne), hv((size_t)ne);
   std::vector<uint8_t> kept((size_t)ne);
   HIPCHK(hipMemcpyAsync(hu.data(), eu, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, stream_));
   HIPCHK(hipMemcpyAsync(hv.data(), ev, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, stream_));
-  rc = sync_read(d.ctr + 8, 1);
+  rc = sync_read(d.ctr + cw::kFlags, 1);
   if (rc) return rc;
   if (host_ctr_[0]) {
     err_ = "edge column without exactly two boundary vertices";
@@ -1176,10 +1502,10 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
   int bits = 1;
   while (bits < 31 && (1ll << bits) < std::max<int64_t>(nv, 2)) bits++;
   if ((int64_t)bits * (maxdim + 1) > 128) return SSQ_OK;  // keys would not fit 128 bits
-  HIPCHK(hipMemsetAsync(d.ctr + 8, 0, sizeof(uint32_t) * 2, stream_));
+  HIPCHK(hipMemsetAsync(d.ctr + cw::kFlags, 0, sizeof(uint32_t) * 2, stream_));
   for (int32_t dd = 0; dd <= maxdim; dd++)
     hipLaunchKernelGGL(k_vertex_sets, dim3(grid_for(n)), dim3(256), 0, stream_, n, dd, d_dims, d_colptr, d_rows,
-                       vrank, vs, d.ctr + 8);
+                       vrank, vs, d.ctr + cw::kFlags);
   // edges (in column order) -> adjacency over vertex ranks, kept edges only
   int32_t* ecols;
   rc = ensure(b_e_, (size_t)(3 * n + 16), &ecols);
@@ -1190,10 +1516,10 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
   int32_t* eu = ecols + n;
   int32_t* ev = ecols + 2 * n;
   hipLaunchKernelGGL(k_edge_ends, dim3(grid_for(std::max<int64_t>(ne, 1))), dim3(256), 0, stream_, ne, ecols,
-                     d_colptr, d_rows, eu, ev, d.ctr + 9);
+                     d_colptr, d_rows, eu, ev, d.ctr + cw::kFlags + 1);
   hipLaunchKernelGGL(k_vrank_ends, dim3(grid_for(std::max<int64_t>(ne, 1))), dim3(256), 0, stream_, ne, vrank, eu,
-                     ev, d.ctr + 9);
-  rc = sync_read(d.ctr + 8, 2);
+                     ev, d.ctr + cw::kFlags + 1);
+  rc = sync_read(d.ctr + cw::kFlags, 2);
   if (rc) return rc;
   if (host_ctr_[0] || host_ctr_[1]) return SSQ_OK;  // not a simplicial complex: no guided clearing
   int64_t* aoff;
@@ -1230,11 +1556,11 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
   HIPCHK(hipMemsetAsync(vals, 0xff, sizeof(int32_t) * hsize, stream_));
   hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(n)), dim3(256), 0, stream_, n, maxdim, d_dims, d.removed, vs, bits,
                      keys, vals, (uint64_t)(hsize - 1));
-  HIPCHK(hipMemsetAsync(d.ctr + 5, 0, sizeof(uint32_t), stream_));
+  HIPCHK(hipMemsetAsync(d.ctr + cw::kHdCleared, 0, sizeof(uint32_t), stream_));
   hipLaunchKernelGGL(k_hd_clear, dim3(grid_for(n)), dim3(256), 0, stream_, n, maxdim, d_dims, d.removed, vs, bits,
-                     aoff, adj, keys, vals, (uint64_t)(hsize - 1), p.max_iteration, hd, d.ctr + 5);
+                     aoff, adj, keys, vals, (uint64_t)(hsize - 1), p.max_iteration, hd, d.ctr + cw::kHdCleared);
   HIPCHK(hipGetLastError());
-  rc = sync_read(d.ctr + 5, 1);
+  rc = sync_read(d.ctr + cw::kHdCleared, 1);
   if (rc) return rc;
   if (st) st->n_cleared_hd += host_ctr_[0];
   *hd_out = hd;
@@ -1299,6 +1625,8 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
 
   Dev d{};
   d.n = n;
+  d.short_max = short_max_;
+  d.long_window = long_window_;
   d.dims = d_dims;
   d.colptr = d_colptr;
   int rc;
@@ -1306,24 +1634,24 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       (rc = ensure(b_cap_, (size_t)n, &d.cap)) || (rc = ensure(b_low_, (size_t)n, &d.low)) ||
       (rc = ensure(b_state_, (size_t)n, &d.state)) || (rc = ensure(b_removed_, (size_t)n, &d.removed)) ||
       (rc = ensure(b_flag_, (size_t)(2 * n), &d.flag)) || (rc = ensure(b_piv_, (size_t)n, &d.piv)) ||
-      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)64, &d.ctr)))
+      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)cw::kWords, &d.ctr)))
     return rc;
   int32_t* U;
   if ((rc = ensure(b_U_, (size_t)n, &U))) return rc;
-  HIPCHK(hipMemsetAsync(d.ctr, 0, 64 * sizeof(uint32_t), stream_));
-  d.stats = (unsigned long long*)(d.ctr + 32);
-  d.arena_used = (unsigned long long*)(d.ctr + 36);
-  d.lvl_cnt = d.ctr + 40;
-  d.err = d.ctr + 44;
+  HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
+  d.stats = (unsigned long long*)(d.ctr + cw::kStats);
+  d.arena_used = (unsigned long long*)(d.ctr + cw::kArena);
+  d.lvl_cnt = d.ctr + cw::kLvl;
+  d.err = d.ctr + cw::kErr;
   HIPCHK(hipMemsetAsync(d.removed, 0, (size_t)n, stream_));
 
   // max dimension (and range check of the cell dimensions)
   int32_t maxdim = 0;
   {
-    hipLaunchKernelGGL(k_maxdim, dim3(grid_for(n)), dim3(256), 0, stream_, n, d_dims, (int32_t*)(d.ctr + 48),
-                       d.ctr + 49);
+    hipLaunchKernelGGL(k_maxdim, dim3(grid_for(n)), dim3(256), 0, stream_, n, d_dims,
+                       (int32_t*)(d.ctr + cw::kMaxDim), d.ctr + cw::kMaxDim + 1);
     HIPCHK(hipGetLastError());
-    rc = sync_read(d.ctr + 48, 2);
+    rc = sync_read(d.ctr + cw::kMaxDim, 2);
     if (rc) return rc;
     if (host_ctr_[1]) {
       err_ = "cell dimension outside [0, 7]";
@@ -1373,7 +1701,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     d.arena_cap = arena_cap;
     HIPCHK(hipMemcpyAsync(d.pool, d_rows, sizeof(int32_t) * std::max<int64_t>(nnz_eff, 0), hipMemcpyDeviceToDevice,
                           stream_));
-    HIPCHK(hipMemsetAsync(d.ctr, 0, 64 * sizeof(uint32_t), stream_));
+    HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
     const int g = grid_for(n);
     hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d);
     hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
@@ -1391,7 +1719,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       upend += cnt[k];
     }
     // page bounds per dimension
-    int32_t* rowstat = (int32_t*)(d.ctr + 16);
+    int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
     {
       std::vector<int32_t> init(2 * kMaxDims);
       for (int k = 0; k < kMaxDims; k++) {
@@ -1400,15 +1728,15 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       }
       HIPCHK(hipMemcpyAsync(rowstat, init.data(), sizeof(int32_t) * 2 * kMaxDims, hipMemcpyHostToDevice, stream_));
       hipLaunchKernelGGL(k_dimstats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
-      rc = sync_read(d.ctr + 4, 16 + 2 * kMaxDims - 4);
+      rc = sync_read(d.ctr, cw::kWords);
       if (rc) return rc;
     }
     int32_t maxrow[kMaxDims], minrow[kMaxDims];
     for (int k = 0; k < kMaxDims; k++) {
-      maxrow[k] = (int32_t)host_ctr_[12 + k];
-      minrow[k] = (int32_t)host_ctr_[12 + kMaxDims + k];
+      maxrow[k] = (int32_t)host_ctr_[cw::kRowStat + k];
+      minrow[k] = (int32_t)host_ctr_[cw::kRowStat + kMaxDims + k];
     }
-    S->n_apparent = host_ctr_[0];
+    S->n_apparent = host_ctr_[cw::kApparent];
     S->n_pending = upend;
     // level slots (specseq.h:281-283), arena-backed
     d.lvl_cap[1] = p.level1;
@@ -1424,12 +1752,13 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
 
     // page loop, dimensions high to low (specseq.h:598-704)
     int32_t* W0;
-    if ((rc = ensure(b_work_, (size_t)(4 * std::max<int64_t>(upend, 1) + 64), &W0))) return rc;
+    if ((rc = ensure(b_work_, (size_t)(5 * std::max<int64_t>(upend, 1) + 64), &W0))) return rc;
     const int64_t wcap = std::max<int64_t>(upend, 1);
-    for (int k = 0; k < 4; k++) d.work[k] = W0 + k * wcap;
+    for (int k = 0; k < 5; k++) d.work[k] = W0 + k * wcap;
     const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * 12);
     const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
     const int rounds_per_sync = 2;
+    const int long_grid = cus_ * 2;
     bool oom = false;
     for (int32_t dim = maxdim; dim >= 1 && !oom; dim--) {
       const int64_t L = cnt[dim];
@@ -1440,7 +1769,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       int64_t mrow = (int64_t)maxrow[dim] - ss, xrow = mrow + ss;
       const int64_t dminrow = minrow[dim];
       while ((begin < L || xrow > dminrow) && !oom) {
-        HIPCHK(hipMemsetAsync(d.ctr, 0, 4 * sizeof(uint32_t), stream_));
+        HIPCHK(hipMemsetAsync(d.ctr, 0, 8 * sizeof(uint32_t), stream_));
         const int64_t wn = end - begin;
         if (wn > 0) {
           const int32_t fl = (int32_t)std::max<int64_t>(mrow, INT32_MIN);
@@ -1451,26 +1780,30 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
           for (;;) {
             for (int t = 0; t < rounds_per_sync; t++, r++) {
               hipLaunchKernelGGL(k_claim, dim3(cg), dim3(256), 0, stream_, d, r);
-              if (timing_) HIPCHK(hipEventRecord(ev_[2 * t], stream_));
+              if (timing_) HIPCHK(hipEventRecord(ev_[4 * t], stream_));
               hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(kWave * kAddWaves), 0, stream_, d, r, fl);
-              if (timing_) HIPCHK(hipEventRecord(ev_[2 * t + 1], stream_));
+              if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 1], stream_));
+              hipLaunchKernelGGL(k_add_long, dim3(long_grid), dim3(kLongThreads), 0, stream_, d, r, fl);
+              if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 2], stream_));
             }
             HIPCHK(hipGetLastError());
-            rc = sync_read(d.ctr, 64);
+            rc = sync_read(d.ctr, cw::kWords);
             if (rc) return rc;
             S->n_add_launches += rounds_per_sync;
             if (timing_)
               for (int t = 0; t < rounds_per_sync; t++) {
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, ev_[2 * t], ev_[2 * t + 1]));
+                float ms = 0.f, ms2 = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev_[4 * t], ev_[4 * t + 1]));
+                HIPCHK(hipEventElapsedTime(&ms2, ev_[4 * t + 1], ev_[4 * t + 2]));
                 S->t_add_kernel_ms += ms;
+                S->t_add_long_kernel_ms += ms2;
               }
             S->n_rounds += rounds_per_sync;
-            if (host_ctr_[44]) {
+            if (host_ctr_[cw::kErr]) {
               oom = true;
               break;
             }
-            if (host_ctr_[r & 1] == 0) break;
+            if (host_ctr_[cw::kW + (r & 1)] == 0) break;
             if (r > (1 << 24)) {
               err_ = "page did not converge";
               return SSQ_EDEVICE;
@@ -1523,13 +1856,18 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       rc = compact(n, 1, KeyDeathOf{death_of}, EmitDeathOf{death_of, d_pairs}, &np2);
       if (rc) return rc;
     }
-    rc = sync_read(d.ctr, 64);
+    rc = sync_read(d.ctr, cw::kWords);
     if (rc) return rc;
-    const unsigned long long* s64 = (const unsigned long long*)(host_ctr_ + 32);
+    const unsigned long long* s64 = (const unsigned long long*)(host_ctr_ + cw::kStats);
     S->n_additions = (int64_t)s64[0];
     S->add_bytes = (double)s64[1];
     S->store_bytes = (int64_t)(nnz_eff + arena_cap) * 4;
-    arena_hint_ = std::max(arena_hint_, (int64_t)(*(const unsigned long long*)(host_ctr_ + 36)) * 2);
+    arena_hint_ = std::max(arena_hint_, (int64_t)(*(const unsigned long long*)(host_ctr_ + cw::kArena)) * 2);
+    S->n_tiled_additions = (int64_t)s64[2];
+    S->n_dense_additions = (int64_t)s64[3];
+    S->dense_bytes = (double)s64[4];
+    S->max_chain = host_ctr_[cw::kChain];
+    S->max_addition_len = host_ctr_[cw::kChain + 1];
     auto T2 = std::chrono::steady_clock::now();
     S->t_reduce_ms = std::chrono::duration<double, std::milli>(T2 - T1).count();
     S->t_total_ms = std::chrono::duration<double, std::milli>(T2 - T0).count();

@@ -169,3 +169,32 @@ def test_invalid_parameters_are_rejected(h):
     for kw in [{"ss_block_size": 0}, {"growth": 0.5}, {"queue_size": 0}, {"level1": 0}, {"model": 4}]:
         with pytest.raises(S.SSQError):
             h.reduce(as_product(mo), S.params(**kw))
+
+
+def _circle(n, seed):
+    rng = np.random.default_rng(seed)
+    t = rng.random(n) * 2 * np.pi
+    X = np.stack([np.cos(t), np.sin(t), np.zeros(n)], 1) + rng.normal(0, 0.05, (n, 3))
+    from scipy.spatial.distance import pdist
+    return X.astype(np.float32), float(np.percentile(pdist(X.astype(np.float32).astype(np.float64)), 30))
+
+
+@pytest.mark.parametrize("env", [{}, {"SSQ_TEST_SHORT_MAX": "16"},
+                                 {"SSQ_TEST_SHORT_MAX": "16", "SSQ_TEST_WINDOW_BITS": "256"},
+                                 {"SSQ_TEST_WINDOW_BITS": "4096"}])
+def test_long_column_paths(env, monkeypatch):
+    """Noisy circles in cohomology grow long cocycle columns: dense LDS bit-set path, the
+    fallback to tiled merges when a pivot leaves the window, and both with forced limits."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    hd = S.Handle(0)
+    try:
+        for seed in range(2):
+            X, r = _circle(150, seed)
+            m = S.vr_build(X, r, 2)
+            mo = O.Matrix(m.colptr, m.rows, m.dims)
+            for kw in [{"dualize": 1}, {"dualize": 0, "model": 1}]:
+                pairs, ess, st = hd.reduce(m, S.params(**kw), stats=True)
+                assert np.array_equal(pairs, O.twist_pairs(mo)), (env, kw, seed)
+    finally:
+        hd.close()

@@ -83,6 +83,35 @@ def split_pairs(pairs, base):
     return out
 
 
+SERVED_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=16384, level1=64, level2=1024)
+
+
+def build_served_pool(seed, n_clouds=48, n_mats=12):
+    """Requests for the served stream and their reference pairs (computed alone on the GPU
+    once here; the oracle checks a sample of them in the parity leg)."""
+    import specseqplusplus_amd as S
+    from specseqplusplus_amd.alpha import alpha_complex
+    rng = np.random.default_rng(seed)
+    pts = []
+    for i in range(n_clouds):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(4096))))
+        X = make_cloud(rng, i % 3, n).astype(np.float32)
+        q = min(30.0, 100.0 * 20.0 / (n - 1))  # 30th percentile, capped at mean degree 20
+        pts.append((X, percentile_threshold(X, q)))
+    mats = []
+    for i in range(n_mats):
+        cols = np.exp(rng.uniform(np.log(1e4), np.log(1e6)))
+        npts = max(64, int(cols / 28.5))
+        P = rng.random((npts, 3)) * 100.0 * (npts / 1e5) ** (1 / 3)
+        mats.append(alpha_complex(P, 90.0))
+    h = S.Handle(0)
+    p = S.params(**SERVED_PARAMS)
+    ref_pts = [h.reduce(S.vr_build(X, r, 2), p)[0] for X, r in pts]
+    ref_mats = [h.reduce(m, p)[0] for m in mats]
+    h.close()
+    return pts, mats, (ref_pts, ref_mats)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,8 +121,10 @@ def main():
     ap.add_argument("--nmin", type=int, default=64)
     ap.add_argument("--nmax", type=int, default=256)
     ap.add_argument("--plain", type=int, default=512, help="complexes timed one call each (plain GPU)")
-    ap.add_argument("--served", type=int, default=4000, help="served-stream requests (0 = skip)")
-    ap.add_argument("--qps", type=float, default=1000.0)
+    ap.add_argument("--served", type=int, default=2000, help="served requests per offered load (0 = skip)")
+    ap.add_argument("--qps", type=float, nargs="+", default=[200.0, 1000.0, 5000.0])
+    ap.add_argument("--server-streams", type=int, default=3)
+    ap.add_argument("--server-cells", type=int, default=2_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
@@ -199,38 +230,50 @@ def main():
         plain = len(dev_mats) / (time.perf_counter() - tt)
         print(f"[bench] plain per-complex GPU: {plain:.0f} diagrams/s", file=sys.stderr, flush=True)
 
-    # served stream: Poisson arrivals of matrices from this stream
+    # served stream: Poisson arrivals; half point clouds (64-4096 points, Rips built by the
+    # server), half prebuilt alpha-complex matrices (1e4-1e6 columns); several offered loads
     served = None
     if args.served > 0:
-        rng = np.random.default_rng(3)
-        picks = rng.integers(0, args.complexes, args.served)
-        gaps = rng.exponential(1.0 / args.qps, args.served)
-        lat = []
-        with Server(streams=3, max_batch=256, p=plans[-1]["p"]) as srv:
-            tickets = []
-            ts = time.perf_counter()
-            nxt = ts
-            for k in range(args.served):
-                nxt += gaps[k]
-                while True:
-                    now = time.perf_counter()
-                    if now >= nxt:
-                        break
-                    time.sleep(min(nxt - now, 0.001))
-                tickets.append(srv.submit_matrix(mats[picks[k]]))
-            srv.wait_all()
-            te = time.perf_counter()
-            exact = True
-            for k, tk in enumerate(tickets):
-                r = srv.poll(tk)
-                lat.append(r[2])
-                if k % 97 == 0:
-                    exact &= bool(np.array_equal(r[0], per[picks[k]]))
-        lat = np.array(lat)
-        served = {"requests": args.served, "qps_offered": args.qps, "diagrams_per_s": args.served / (te - ts),
-                  "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                  "bit_exact_vs_batched": exact}
-        print(f"[bench] served {served}", file=sys.stderr, flush=True)
+        served = {"requests_per_load": args.served, "loads": []}
+        pool_pts, pool_mats, pool_ref = build_served_pool(args.seed + 7)
+        print(f"[bench] served pool: {len(pool_pts)} clouds, {len(pool_mats)} matrices "
+              f"({min(m.n for m in pool_mats)}-{max(m.n for m in pool_mats)} cells)", file=sys.stderr, flush=True)
+        for qps in args.qps:
+            rng = np.random.default_rng(int(qps))
+            kinds = rng.integers(0, 2, args.served)
+            picks = rng.integers(0, 1 << 30, args.served)
+            gaps = rng.exponential(1.0 / qps, args.served)
+            lat = []
+            with Server(streams=args.server_streams, max_batch=64, max_batch_cells=args.server_cells,
+                        p=S.params(**SERVED_PARAMS)) as srv:
+                tickets = []
+                ts = time.perf_counter()
+                nxt = ts
+                for k in range(args.served):
+                    nxt += gaps[k]
+                    while True:
+                        now = time.perf_counter()
+                        if now >= nxt:
+                            break
+                        time.sleep(min(nxt - now, 0.0005))
+                    if kinds[k] == 0:
+                        X, r = pool_pts[picks[k] % len(pool_pts)]
+                        tickets.append((0, picks[k] % len(pool_pts), srv.submit_points(X, r, 2)))
+                    else:
+                        tickets.append((1, picks[k] % len(pool_mats), srv.submit_matrix(pool_mats[picks[k] % len(pool_mats)])))
+                srv.wait_all()
+                te = time.perf_counter()
+                exact = True
+                for kind, j, tk in tickets:
+                    res = srv.poll(tk)
+                    lat.append(res[2])
+                    exact &= bool(np.array_equal(res[0], pool_ref[kind][j]))
+            lat = np.array(lat)
+            rec = {"qps_offered": qps, "diagrams_per_s": round(args.served / (te - ts), 1),
+                   "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "bit_exact": exact}
+            served["loads"].append(rec)
+            print(f"[bench] served {rec}", file=sys.stderr, flush=True)
 
     st = {k: sum(x[k] for x in stats[-len(plans):]) for k in stats[-1] if isinstance(stats[-1][k], (int, float))}
     add_ms = sum(s["t_add_kernel_ms"] for s in stats)

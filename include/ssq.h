@@ -135,6 +135,7 @@ typedef struct ssq_server_config {
   int device;
   int streams;               /* batches that may be on the device at once (>=1)        */
   int max_batch;             /* requests per batch                                     */
+  int64_t max_batch_cells;   /* cells per batch (0 = no limit): bounds batch latency   */
   int64_t mem_budget_bytes;  /* admission budget of estimated device bytes (0 = auto) */
   ssq_params params;
 } ssq_server_config;

@@ -969,12 +969,45 @@ __global__ void k_scan_final(const int32_t* in, int64_t n, const int64_t* bsum, 
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
 }
 
-__global__ void k_dual_pairs(int64_t np, const int64_t* dpairs, int64_t n, int32_t* death_of) {
-  GRID_STRIDE(i, np) death_of[n - 1 - dpairs[2 * i + 1]] = (int32_t)(n - 1 - dpairs[2 * i]);
+// death_of[b] for the primal pairing: every dual pair (r, piv[r]) is the primal pair
+// (n-1-piv[r], n-1-r); death_of is -1 beforehand.
+__global__ void k_dual_pairs_fill(int64_t n, const uint32_t* piv, int32_t* death_of) {
+  GRID_STRIDE(r, n) {
+    const uint32_t c = piv[r];
+    if (c != kEmpty) death_of[n - 1 - (int64_t)c] = (int32_t)(n - 1 - r);
+  }
 }
 
-__global__ void k_dual_essential(int64_t ne, const int64_t* in, int64_t n, int64_t* out) {
+__global__ void k_dual_essential(const int64_t* ne_dev, const int64_t* in, int64_t n, int64_t* out) {
+  const int64_t ne = *ne_dev;
   GRID_STRIDE(k, ne) out[ne - 1 - k] = n - 1 - in[k];
+}
+
+__global__ void k_rowstat_init(int32_t* rowstat) {
+  if (threadIdx.x < kMaxDims) {
+    rowstat[threadIdx.x] = -1;
+    rowstat[kMaxDims + threadIdx.x] = 0x7fffffff;
+  }
+}
+
+
+// Block-diagonal batch: complex i's cells are shifted by base[i] and its nonzeros by
+// nzb[i]; the host only concatenates the arrays unshifted.
+__device__ __forceinline__ int64_t find_block(const int64_t* off, int64_t count, int64_t x) {
+  int64_t lo = 0, hi = count;  // largest i with off[i] <= x
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_batch_shift(int64_t n, int64_t nnz, int64_t count, const int64_t* base, const int64_t* nzb,
+                              int64_t* colptr, int32_t* rows) {
+  GRID_STRIDE(c, n) colptr[c] += nzb[find_block(base, count, c)];
+  GRID_STRIDE(k, nnz) rows[k] += (int32_t)base[find_block(nzb, count, k)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) colptr[n] = nnz;
 }
 
 // ------------------------------------------------------------------------------------
@@ -1258,6 +1291,8 @@ class Engine {
     HIPCHK(hipSetDevice(device_));
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc(&host_ctr_, cw::kWords * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&host_tot_, kTotSlots * sizeof(int64_t), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&tot_, kTotSlots * sizeof(int64_t)));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_));
     cus_ = prop.multiProcessorCount;
@@ -1306,6 +1341,8 @@ class Engine {
                       &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
+    if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
+    if (tot_) (void)hipFree(tot_), tot_ = nullptr;
     for (auto& e : ev_)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
@@ -1320,8 +1357,17 @@ class Engine {
     const int64_t cap = (int64_t)cus_ * 16;
     return (int)std::max<int64_t>(1, std::min(g, cap));
   }
+  // Stable partition by a small key; totals go to device slot tot_[slot .. slot+nkeys).
+  // With totals_host, they are also copied back (one sync); without, nothing syncs.
   template <typename KeyF, typename EmitF>
-  int compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host);
+  int compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host, int slot = kTotScratch);
+  // read counters and all total slots in one round trip
+  int sync_all() {
+    HIPCHK(hipMemcpyAsync(host_ctr_, ctr_dev_, sizeof(uint32_t) * cw::kWords, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(host_tot_, tot_, sizeof(int64_t) * kTotSlots, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return SSQ_OK;
+  }
   int collapse(Dev& d, int64_t n, const int32_t* d_dims, const int64_t* d_colptr, const int32_t* d_rows,
                ssq_stats* st);
   int guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dims, const int64_t* d_colptr,
@@ -1329,6 +1375,16 @@ class Engine {
   int scan_counts(const int32_t* cnt, int64_t n, int64_t* out);
   int build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
                  const int32_t* dims, int64_t** dcolptr, int32_t** drows, int32_t** ddims);
+
+  static constexpr int kTotPending = 0;   // [0..7] pending columns per dimension
+  static constexpr int kTotPairs = 8;     // pairs
+  static constexpr int kTotEss = 9;       // essential cells
+  static constexpr int kTotDualPairs = 10;
+  static constexpr int kTotScratch = 16;  // [16..23] counts only needed right away
+  static constexpr int kTotSlots = 24;
+  int64_t* tot_ = nullptr;       // device
+  int64_t* host_tot_ = nullptr;  // pinned
+  uint32_t* ctr_dev_ = nullptr;
 
   int device_ = 0;
   int cus_ = 256;
@@ -1346,12 +1402,12 @@ class Engine {
 };
 
 template <typename KeyF, typename EmitF>
-int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host) {
+int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host, int slot) {
   const int64_t nb = std::max<int64_t>(1, (n + kCompactBlock - 1) / kCompactBlock);
   uint32_t* counts;
-  int rc = ensure(b_cnt_, (size_t)(nb * nkeys + 2 * kMaxDims + 16), &counts);
+  int rc = ensure(b_cnt_, (size_t)(nb * nkeys + 16), &counts);
   if (rc) return rc;
-  int64_t* totals = (int64_t*)(counts + nb * nkeys + (nb * nkeys) % 2);
+  int64_t* totals = tot_ + slot;
   if (n > 0) {
     hipLaunchKernelGGL(k_key_count<KeyF>, dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key, counts,
                        nkeys);
@@ -1363,9 +1419,10 @@ int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_
     hipLaunchKernelGGL((k_key_scatter<KeyF, EmitF>), dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key,
                        counts, nkeys, emit);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(host_ctr_, totals, sizeof(int64_t) * nkeys, hipMemcpyDeviceToHost, stream_));
+  if (!totals_host) return SSQ_OK;  // the caller reads tot_[slot..] at its next sync
+  HIPCHK(hipMemcpyAsync(host_tot_ + slot, totals, sizeof(int64_t) * nkeys, hipMemcpyDeviceToHost, stream_));
   HIPCHK(hipStreamSynchronize(stream_));
-  std::memcpy(totals_host, host_ctr_, sizeof(int64_t) * nkeys);
+  std::memcpy(totals_host, host_tot_ + slot, sizeof(int64_t) * nkeys);
   return SSQ_OK;
 }
 
@@ -1639,6 +1696,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
   int32_t* U;
   if ((rc = ensure(b_U_, (size_t)n, &U))) return rc;
   HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
+  ctr_dev_ = d.ctr;
   d.stats = (unsigned long long*)(d.ctr + cw::kStats);
   d.arena_used = (unsigned long long*)(d.ctr + cw::kArena);
   d.lvl_cnt = d.ctr + cw::kLvl;
@@ -1682,9 +1740,11 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     int32_t *drw, *ddm;
     rc = build_dual(d, n, nnz, maxdim, d_colptr, d_rows, d_dims, &dcp, &drw, &ddm);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(host_ctr_, dcp + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
-    std::memcpy(&nnz_eff, host_ctr_, sizeof(int64_t));
+    if (p.model & 2) {  // removed cells drop their entries from the dual
+      HIPCHK(hipMemcpyAsync(host_tot_ + kTotScratch, dcp + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+      HIPCHK(hipStreamSynchronize(stream_));
+      nnz_eff = host_tot_[kTotScratch];
+    }
     d_colptr = dcp;
     d_rows = drw;
     d_dims = ddm;
@@ -1709,27 +1769,21 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     hipLaunchKernelGGL(k_clearing, dim3(g), dim3(256), 0, stream_, d);
     hipLaunchKernelGGL(k_compress, dim3(g), dim3(256), 0, stream_, d);
     HIPCHK(hipGetLastError());
-    int64_t cnt[kMaxDims] = {0};
-    rc = compact(n, kMaxDims, KeyPendingDim{d.state, d.dims}, EmitIndex{U}, cnt);
+    // pending lists per dimension and page bounds, read back in one round trip
+    rc = compact(n, kMaxDims, KeyPendingDim{d.state, d.dims}, EmitIndex{U}, nullptr, kTotPending);
     if (rc) return rc;
+    int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
+    hipLaunchKernelGGL(k_rowstat_init, dim3(1), dim3(64), 0, stream_, rowstat);
+    hipLaunchKernelGGL(k_dimstats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
+    rc = sync_all();
+    if (rc) return rc;
+    int64_t cnt[kMaxDims];
     int64_t ubase[kMaxDims];
     int64_t upend = 0;
     for (int k = 0; k < kMaxDims; k++) {
+      cnt[k] = host_tot_[kTotPending + k];
       ubase[k] = upend;
       upend += cnt[k];
-    }
-    // page bounds per dimension
-    int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
-    {
-      std::vector<int32_t> init(2 * kMaxDims);
-      for (int k = 0; k < kMaxDims; k++) {
-        init[k] = -1;
-        init[kMaxDims + k] = 0x7fffffff;
-      }
-      HIPCHK(hipMemcpyAsync(rowstat, init.data(), sizeof(int32_t) * 2 * kMaxDims, hipMemcpyHostToDevice, stream_));
-      hipLaunchKernelGGL(k_dimstats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
-      rc = sync_read(d.ctr, cw::kWords);
-      if (rc) return rc;
     }
     int32_t maxrow[kMaxDims], minrow[kMaxDims];
     for (int k = 0; k < kMaxDims; k++) {
@@ -1811,6 +1865,8 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
           }
         }
         S->n_pages++;
+        // a page that covered every column with every row active finished the dimension
+        if (end >= L && mrow <= dminrow) break;
         ss = (int32_t)((float)ss * p.growth);
         if (ss < 1) ss = 1;
         if (xrow <= dminrow) begin = end;
@@ -1833,31 +1889,30 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       S->n_pages = S->n_rounds = 0;
       continue;  // rerun from the matrix with a larger arena
     }
-    // pairs in row order
-    int64_t np = 0;
-    rc = compact(n, 1, KeyPivot{d.piv}, EmitPair{d.piv, d_pairs}, &np);
-    if (rc) return rc;
-    *np_out = np;
-    if (d_essential && ne_out) {
-      int64_t ne = 0;
-      int64_t* ess_dst = d_essential;
-      if (dual && (rc = ensure(b_keys_, (size_t)n, &ess_dst))) return rc;
-      rc = compact(n, 1, KeyEssential{d.piv, d.removed, d.state, d.low}, EmitI64{ess_dst}, &ne);
+    // pairs in row order, essential cells; one round trip at the end
+    if (!dual) {
+      rc = compact(n, 1, KeyPivot{d.piv}, EmitPair{d.piv, d_pairs}, nullptr, kTotPairs);
       if (rc) return rc;
-      if (dual && ne)
-        hipLaunchKernelGGL(k_dual_essential, dim3(grid_for(ne)), dim3(256), 0, stream_, ne, ess_dst, n, d_essential);
-      *ne_out = ne;
-    }
-    if (dual) {  // (b', d') -> (n-1-d', n-1-b'), sorted by the new birth
+    } else {  // dual pair (b', d') -> (n-1-d', n-1-b'), sorted by the new birth
       int32_t* death_of = (int32_t*)d.leftone;
       HIPCHK(hipMemsetAsync(death_of, 0xff, sizeof(int32_t) * n, stream_));
-      if (np) hipLaunchKernelGGL(k_dual_pairs, dim3(grid_for(np)), dim3(256), 0, stream_, np, d_pairs, n, death_of);
-      int64_t np2 = 0;
-      rc = compact(n, 1, KeyDeathOf{death_of}, EmitDeathOf{death_of, d_pairs}, &np2);
+      hipLaunchKernelGGL(k_dual_pairs_fill, dim3(grid_for(n)), dim3(256), 0, stream_, n, d.piv, death_of);
+      rc = compact(n, 1, KeyDeathOf{death_of}, EmitDeathOf{death_of, d_pairs}, nullptr, kTotPairs);
       if (rc) return rc;
     }
-    rc = sync_read(d.ctr, cw::kWords);
+    if (d_essential && ne_out) {
+      int64_t* ess_dst = d_essential;
+      if (dual && (rc = ensure(b_keys_, (size_t)n, &ess_dst))) return rc;
+      rc = compact(n, 1, KeyEssential{d.piv, d.removed, d.state, d.low}, EmitI64{ess_dst}, nullptr, kTotEss);
+      if (rc) return rc;
+      if (dual)
+        hipLaunchKernelGGL(k_dual_essential, dim3(grid_for(n)), dim3(256), 0, stream_, tot_ + kTotEss, ess_dst, n,
+                           d_essential);
+    }
+    rc = sync_all();
     if (rc) return rc;
+    *np_out = host_tot_[kTotPairs];
+    if (d_essential && ne_out) *ne_out = host_tot_[kTotEss];
     const unsigned long long* s64 = (const unsigned long long*)(host_ctr_ + cw::kStats);
     S->n_additions = (int64_t)s64[0];
     S->add_bytes = (double)s64[1];
@@ -1881,7 +1936,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
 struct Handle {
   Engine eng;
   std::string err;
-  DevBuf in_colptr, in_rows, in_dims, out_pairs, out_ess;
+  DevBuf in_colptr, in_rows, in_dims, out_pairs, out_ess, offs;
   explicit Handle(int dev) : eng(dev) {}
 };
 
@@ -2004,7 +2059,7 @@ extern "C" int ssq_create(int device, ssq_handle** out) {
 
 extern "C" int ssq_destroy(ssq_handle* h) {
   if (!h) return SSQ_EINVAL;
-  for (DevBuf* b : {&h->h->in_colptr, &h->h->in_rows, &h->h->in_dims, &h->h->out_pairs, &h->h->out_ess})
+  for (DevBuf* b : {&h->h->in_colptr, &h->h->in_rows, &h->h->in_dims, &h->h->out_pairs, &h->h->out_ess, &h->h->offs})
     if (b->p) (void)hipFree(b->p);
   delete h->h;
   delete h;
@@ -2060,17 +2115,60 @@ extern "C" int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, in
     h->h->err = "batch exceeds 2^31 cells";
     return SSQ_EINVAL;
   }
+  // unshifted concatenation on the host (memcpy only); offsets are applied on the device
   std::vector<int64_t> cp((size_t)N + 1);
-  std::vector<int32_t> rw((size_t)NNZ), dm((size_t)N);
+  std::vector<int32_t> rw((size_t)std::max<int64_t>(NNZ, 1)), dm((size_t)std::max<int64_t>(N, 1));
   for (int i = 0; i < count; i++) {
     const ssq_matrix* m = mats[i];
-    for (int64_t c = 0; c <= m->n; c++) cp[(size_t)(base[i] + c)] = nzb[i] + m->col_ptr[c];
-    for (int64_t k = 0; k < m->nnz; k++) rw[(size_t)(nzb[i] + k)] = (int32_t)(m->rows[k] + base[i]);
+    if (m->n) std::memcpy(&cp[(size_t)base[i]], m->col_ptr, sizeof(int64_t) * m->n);
+    if (m->nnz) std::memcpy(&rw[(size_t)nzb[i]], m->rows, sizeof(int32_t) * m->nnz);
     if (m->n) std::memcpy(&dm[(size_t)base[i]], m->dims, sizeof(int32_t) * m->n);
   }
+  cp[(size_t)N] = 0;
   std::vector<int64_t> pairs, ess;
-  int rc = reduce_host(h->h, N, NNZ, cp.data(), rw.data(), dm.data(), *p, pairs, ess, stats);
+  int64_t *dc, *dp, *de;
+  int32_t *dr, *dd;
+  int rc = upload(h->h, cp.data(), rw.data(), dm.data(), N, NNZ, &dc, &dr, &dd, &dp, &de);
   if (rc) return rc;
+  {
+    DevBuf& ob = h->h->offs;
+    const size_t bytes = sizeof(int64_t) * 2 * ((size_t)count + 1);
+    if (ob.bytes < bytes) {
+      if (ob.p) (void)hipFree(ob.p);
+      ob.p = nullptr;
+      ob.bytes = 0;
+      if (hipMalloc(&ob.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return SSQ_ENOMEM;
+      }
+      ob.bytes = bytes;
+    }
+    int64_t* dbase = (int64_t*)ob.p;
+    int64_t* dnzb = dbase + count + 1;
+    hipStream_t st = h->h->eng.stream();
+    if (hipMemcpyAsync(dbase, base.data(), sizeof(int64_t) * (count + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dnzb, nzb.data(), sizeof(int64_t) * (count + 1), hipMemcpyHostToDevice, st) != hipSuccess)
+      return SSQ_EDEVICE;
+    const int64_t work = std::max<int64_t>(N, NNZ);
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_batch_shift, dim3(grid), dim3(256), 0, st, N, NNZ, (int64_t)count, dbase, dnzb, dc, dr);
+    if (hipGetLastError() != hipSuccess) return SSQ_EDEVICE;
+  }
+  int64_t np = 0, ne = 0;
+  rc = h->h->eng.reduce_device(N, NNZ, dc, dr, dd, *p, dp, &np, stats, de, &ne);
+  if (rc) {
+    h->h->err = h->h->eng.error();
+    return rc;
+  }
+  pairs.resize((size_t)(2 * np));
+  ess.resize((size_t)ne);
+  {
+    hipStream_t st = h->h->eng.stream();
+    if ((np && hipMemcpyAsync(pairs.data(), dp, sizeof(int64_t) * 2 * np, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        (ne && hipMemcpyAsync(ess.data(), de, sizeof(int64_t) * ne, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return SSQ_EDEVICE;
+  }
   // pairs are sorted by birth, so each complex's pairs are a contiguous run
   size_t pi = 0, ei = 0;
   for (int i = 0; i < count; i++) {

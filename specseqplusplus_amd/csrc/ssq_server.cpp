@@ -36,6 +36,7 @@ struct Request {
   float r = 0;
   bool is_points = false;
   int64_t est_bytes = 0;
+  int64_t est_cells = 0;
 };
 
 struct Result {
@@ -85,10 +86,13 @@ struct ssq_server {
         if (stop && queue.empty()) break;
         // admission: oldest first while it fits the batch size and the device budget; the
         // first request is always admitted when nothing else is in flight
+        int64_t cells = 0;
         while (!queue.empty() && (int)batch.size() < cfg.max_batch) {
           Request* q = queue.front();
           if (in_flight_bytes + q->est_bytes > cfg.mem_budget_bytes && (in_flight_bytes > 0 || !batch.empty()))
             break;
+          if (cfg.max_batch_cells > 0 && !batch.empty() && cells + q->est_cells > cfg.max_batch_cells) break;
+          cells += q->est_cells;
           queue.pop_front();
           in_flight_bytes += q->est_bytes;
           batch.push_back(q);
@@ -203,6 +207,7 @@ extern "C" int ssq_server_submit_matrix(ssq_server* s, const ssq_matrix* m, int6
   if (m->nnz) q->rows.assign(m->rows, m->rows + m->nnz);
   q->dims.assign(m->dims, m->dims + m->n);
   q->est_bytes = estimate_bytes(m->n, m->nnz);
+  q->est_cells = m->n;
   return s->submit(q, ticket);
 }
 
@@ -218,6 +223,7 @@ extern "C" int ssq_server_submit_points(ssq_server* s, const float* pts, int64_t
   q->r = r;
   q->maxdim = maxdim;
   q->est_bytes = estimate_points(npts, maxdim);
+  q->est_cells = npts * 16 * (maxdim >= 2 ? 10 : 1);
   return s->submit(q, ticket);
 }
 

@@ -8,9 +8,10 @@ from .api import params as make_params
 
 
 class Server:
-    def __init__(self, device=0, streams=2, max_batch=64, mem_budget_bytes=0, p=None):
+    def __init__(self, device=0, streams=2, max_batch=64, mem_budget_bytes=0, p=None, max_batch_cells=0):
         cfg = L.ssq_server_config()
         cfg.device, cfg.streams, cfg.max_batch, cfg.mem_budget_bytes = device, streams, max_batch, mem_budget_bytes
+        cfg.max_batch_cells = max_batch_cells
         cfg.params = p or make_params()
         self._s = ctypes.c_void_p()
         L.check(L.lib().ssq_server_create(ctypes.byref(cfg), ctypes.byref(self._s)), "ssq_server_create")

@@ -157,6 +157,9 @@ int ssq_server_destroy(ssq_server* s);
  * ids) with filtration = longest edge. */
 int ssq_vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim,
                  ssq_matrix* out);
+/* The same matrix built on the handle's GPU (maxdim <= 2), copied back to the host. */
+int ssq_vr_build_gpu(ssq_handle* h, const float* pts, int64_t npts, int pdim, float r, int maxdim,
+                     ssq_matrix* out);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,8 @@ SIGNATURES = {
     "ssq_device_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "ssq_vr_build": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
+    "ssq_vr_build_gpu": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int,
+                                        ctypes.c_float, ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
 }
 
 _lib = None

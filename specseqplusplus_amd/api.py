@@ -123,6 +123,16 @@ class Handle:
                                           ctypes.byref(np_), ctypes.byref(st)), self.error())
         return np_.value, st.as_dict()
 
+    def vr_build_gpu(self, points, r, maxdim=2):
+        """Vietoris-Rips matrix built on this handle's GPU (same cells and order as vr_build)."""
+        pts = np.ascontiguousarray(points, dtype=np.float32)
+        m = L.ssq_matrix()
+        L.check(L.lib().ssq_vr_build_gpu(self._h, pts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), pts.shape[0],
+                                         pts.shape[1], float(r), int(maxdim), ctypes.byref(m)), self.error())
+        out = Matrix.from_c(m)
+        L.lib().ssq_matrix_free(ctypes.byref(m))
+        return out
+
     # ---- device-resident buffers (HBM), owned by the caller through this handle ----
     def device_array(self, arr):
         """Copy a numpy array into a new device buffer; returns a DeviceArray."""

@@ -1937,7 +1937,11 @@ struct Handle {
   Engine eng;
   std::string err;
   DevBuf in_colptr, in_rows, in_dims, out_pairs, out_ess, offs;
+  VrBuilder* vr = nullptr;
   explicit Handle(int dev) : eng(dev) {}
+  ~Handle() {
+    if (vr) vr_builder_destroy(vr);
+  }
 };
 
 static int upload(Handle* H, const int64_t* colptr, const int32_t* rows, const int32_t* dims, int64_t n,
@@ -2227,4 +2231,29 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
   if (!h) return SSQ_EINVAL;
   if (hipSetDevice(h->h->eng.device()) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return SSQ_EDEVICE;
   return SSQ_OK;
+}
+
+extern "C" int ssq_vr_build_gpu(ssq_handle* h, const float* pts, int64_t npts, int pdim, float r, int maxdim,
+                                ssq_matrix* out) {
+  if (!h || !out) return SSQ_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  if (!h->h->vr && !(h->h->vr = vr_builder_create())) return SSQ_ENOMEM;
+  if (hipSetDevice(h->h->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  const int64_t* dc;
+  const int32_t *dr, *dd;
+  int64_t n = 0, nnz = 0;
+  int rc = vr_builder_build(h->h->vr, h->h->eng.stream(), pts, npts, pdim, r, maxdim, &dc, &dr, &dd, &n, &nnz);
+  if (rc) {
+    h->h->err = "GPU Rips build failed";
+    return rc;
+  }
+  std::vector<int64_t> cp((size_t)n + 1);
+  std::vector<int32_t> rw((size_t)nnz), dm((size_t)n);
+  hipStream_t s = h->h->eng.stream();
+  if (hipMemcpyAsync(cp.data(), dc, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      (nnz && hipMemcpyAsync(rw.data(), dr, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (n && hipMemcpyAsync(dm.data(), dd, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return SSQ_EDEVICE;
+  return matrix_from_vectors(cp, rw, dm, out);
 }

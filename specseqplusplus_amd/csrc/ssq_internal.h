@@ -1,5 +1,7 @@
 // Internal declarations shared by the libssq translation units.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -17,5 +19,14 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
 
 // Vietoris-Rips boundary matrix (host builder).
 int vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim, ssq_matrix* out);
+
+// GPU Vietoris-Rips builder (dimension <= 2).  Device outputs stay valid until the next
+// build on the same builder.
+struct VrBuilder;
+VrBuilder* vr_builder_create();
+void vr_builder_destroy(VrBuilder* b);
+int vr_builder_build(VrBuilder* b, hipStream_t s, const float* h_pts, int64_t npts, int pdim, float r, int maxdim,
+                     const int64_t** d_colptr, const int32_t** d_rows, const int32_t** d_dims, int64_t* n_out,
+                     int64_t* nnz_out);
 
 }  // namespace ssq

@@ -198,3 +198,24 @@ def test_long_column_paths(env, monkeypatch):
                 assert np.array_equal(pairs, O.twist_pairs(mo)), (env, kw, seed)
     finally:
         hd.close()
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_gpu_rips_builder_matches_host(h, case):
+    from specseqplusplus_amd import configs as CF
+    rng = np.random.default_rng(case)
+    if case < 3:
+        X = CF.cloud(rng, case, 180 + 40 * case).astype(np.float32)
+        from scipy.spatial.distance import pdist
+        r = float(np.percentile(pdist(X.astype(np.float64)), 30))
+    elif case == 3:
+        X, r = CF.haar_rotations(600, 3), 1.0
+    elif case == 4:
+        X, r = rng.random((1, 3)).astype(np.float32), 0.5   # a single point
+    else:
+        X, r = rng.random((50, 3)).astype(np.float32), 0.0  # no edges
+    for maxdim in (0, 1, 2):
+        a = S.vr_build(X, r, maxdim)
+        b = h.vr_build_gpu(X, r, maxdim)
+        assert np.array_equal(a.dims, b.dims) and np.array_equal(a.colptr, b.colptr), (case, maxdim)
+        assert np.array_equal(a.rows, b.rows), (case, maxdim)

@@ -84,6 +84,8 @@ def split_pairs(pairs, base):
 
 
 SERVED_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=16384, level1=64, level2=1024)
+SERVED_MATRIX_PARAMS = dict(dualize=0, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=1000,
+                            level2=16000)
 
 
 def build_served_pool(seed, n_clouds=48, n_mats=12):
@@ -107,7 +109,7 @@ def build_served_pool(seed, n_clouds=48, n_mats=12):
     h = S.Handle(0)
     p = S.params(**SERVED_PARAMS)
     ref_pts = [h.reduce(S.vr_build(X, r, 2), p)[0] for X, r in pts]
-    ref_mats = [h.reduce(m, p)[0] for m in mats]
+    ref_mats = [h.reduce(m, S.params(**SERVED_MATRIX_PARAMS))[0] for m in mats]
     h.close()
     return pts, mats, (ref_pts, ref_mats)
 
@@ -245,7 +247,7 @@ def main():
             gaps = rng.exponential(1.0 / qps, args.served)
             lat = []
             with Server(streams=args.server_streams, max_batch=64, max_batch_cells=args.server_cells,
-                        p=S.params(**SERVED_PARAMS)) as srv:
+                        p=S.params(**SERVED_MATRIX_PARAMS), points_p=S.params(**SERVED_PARAMS)) as srv:
                 tickets = []
                 ts = time.perf_counter()
                 nxt = ts

@@ -137,7 +137,8 @@ typedef struct ssq_server_config {
   int max_batch;             /* requests per batch                                     */
   int64_t max_batch_cells;   /* cells per batch (0 = no limit): bounds batch latency   */
   int64_t mem_budget_bytes;  /* admission budget of estimated device bytes (0 = auto) */
-  ssq_params params;
+  ssq_params params;         /* reduction of submitted matrices                        */
+  ssq_params points_params;  /* reduction of point-cloud (Rips) requests               */
 } ssq_server_config;
 int ssq_server_create(const ssq_server_config* cfg, ssq_server** out);
 /* Submit a matrix (copied) or a point cloud (npts x pdim float32, Rips threshold r,

@@ -46,7 +46,8 @@ class ssq_stats(ctypes.Structure):
 
 class ssq_server_config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("streams", ctypes.c_int), ("max_batch", ctypes.c_int),
-                ("max_batch_cells", ctypes.c_int64), ("mem_budget_bytes", ctypes.c_int64), ("params", ssq_params)]
+                ("max_batch_cells", ctypes.c_int64), ("mem_budget_bytes", ctypes.c_int64), ("params", ssq_params),
+                ("points_params", ssq_params)]
 
 
 # every symbol declared in include/ssq.h: name -> (restype, argtypes)

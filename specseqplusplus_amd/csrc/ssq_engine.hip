@@ -44,8 +44,17 @@ constexpr int kTile = 1024;            // LDS entries per staging buffer
 constexpr int kTileBuf = kTile + kWave;
 constexpr int kMaxDims = 8;            // cell dimensions 0..7
 
+// Where the pivot column of a row is stored, written once the pivot is established so that
+// an addition can fetch the pivot's id and its storage in one round trip.
+struct __align__(16) PivMeta {
+  int64_t off;    // start of the pivot column in the column store
+  int32_t len;    // its length
+  int32_t first;  // its smallest row
+};
+
 struct Dev {
   int64_t n;
+  PivMeta* pivmeta;    // row -> storage of piv[row]
   const int32_t* dims;
   const int64_t* colptr;
   int32_t* pool;       // column store: [input rows | arena]
@@ -169,7 +178,7 @@ __global__ void k_leftone(Dev d) {
 // column is positive (its column reduces to zero; flag[row] = 1); a column that is leftmost
 // in its low row is an apparent pair; a column leftmost in any row is negative
 // (flag[n + col] = 1).  The two flags live in separate bytes: plain stores of 1 only.
-__global__ void k_lookup(Dev d, const uint8_t* hdclear) {
+__global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
   GRID_STRIDE(c, d.n) {
     const int32_t l = d.len[c];
     if (l == 0) continue;
@@ -180,13 +189,30 @@ __global__ void k_lookup(Dev d, const uint8_t* hdclear) {
       d.state[c] = ST_PIVOT;
       atomicAdd(&d.ctr[cw::kApparent], 1u);
     }
+    if (neg_known) continue;
     const int32_t* col = d.pool + d.off[c];
     for (int32_t k = 0; k < l; k++)
       if (d.leftone[col[k]] == (uint32_t)c) {
         d.flag[d.n + c] = 1;
         break;
       }
-    if (hdclear && hdclear[c]) d.flag[c] = 1;
+  }
+  if (hdclear)
+    GRID_STRIDE(c, d.n) if (hdclear[c]) d.flag[c] = 1;
+}
+
+
+// Dual init from the primal (no pass over the dual's nonzeros): for a primal cell c with
+// boundary, the dual row n-1-c has its leftmost one in dual column n-1-low(c), and the dual
+// column n-1-low(c) is leftmost in that row, i.e. negative (Lemma L in the dual).
+__global__ void k_dual_init_from_primal(int64_t n, const int64_t* colptr, const int32_t* rows,
+                                        const uint8_t* removed_primal, uint32_t* leftone, uint8_t* negflag) {
+  GRID_STRIDE(c, n) {
+    const int64_t e = colptr[c + 1];
+    if (removed_primal[c] || e == colptr[c]) continue;
+    const int32_t low = rows[e - 1];
+    leftone[n - 1 - c] = (uint32_t)(n - 1 - low);
+    negflag[n - 1 - low] = 1;
   }
 }
 
@@ -215,6 +241,22 @@ __global__ void k_compress(Dev d) {
     d.len[c] = o;
     d.low[c] = o ? col[o - 1] : -1;
     if (!o) d.state[c] = ST_ZERO;
+    else if (d.state[c] == ST_PIVOT) d.pivmeta[col[o - 1]] = PivMeta{d.off[c], o, col[0]};
+  }
+}
+
+// After a claim round: the winners publish where their columns are stored.
+__global__ void k_pivmeta(Dev d, int r) {
+  const int in = r & 1;
+  const uint32_t cnt = d.ctr[cw::kW + in];
+  const int32_t* list = d.work[in];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = list[i];
+    const int32_t l = d.low[c];
+    if (d.piv[l] == (uint32_t)c) {
+      const int64_t o = d.off[c];
+      d.pivmeta[l] = PivMeta{o, d.len[c], d.pool[o]};
+    }
   }
 }
 
@@ -462,9 +504,10 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
     bool deferred = false;
     for (;;) {
       const uint32_t p = d.piv[l];
+      const PivMeta pm = d.pivmeta[l];
       if (p == kEmpty || p >= (uint32_t)c) break;
-      const int32_t b_len = d.len[p];
-      const int32_t* B = d.pool + d.off[p];
+      const int32_t b_len = pm.len;
+      const int32_t* B = d.pool + pm.off;
       int32_t r_len;
       if (a_len + b_len <= d.short_max) {
         if (!in_lds) {
@@ -562,7 +605,7 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
 // window sends the column back to sorted form and tiled merges.
 constexpr int kLongBits = 1 << 19;
 constexpr int kLongWords = kLongBits / 32;
-constexpr int kLongThreads = 256;
+constexpr int kLongThreads = 512;
 
 __device__ __forceinline__ int32_t block_max_i32(int32_t v, int32_t* scratch) {
   v = wave_max(v);
@@ -627,11 +670,17 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
       __syncthreads();
     }
     for (;;) {
-      const uint32_t p = d.piv[l];
+      // the chase state is uniform across the workgroup: scalar loads
+      l = __builtin_amdgcn_readfirstlane(l);
+      const uint32_t p = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.piv[l]);
+      const PivMeta pm = d.pivmeta[l];
       if (p == kEmpty || p >= (uint32_t)c) break;
-      const int32_t b_len = d.len[p];
-      const int32_t* B = d.pool + d.off[p];
-      if (dense && B[0] < lo) {
+      const int32_t b_len = __builtin_amdgcn_readfirstlane(pm.len);
+      const int32_t b_first = __builtin_amdgcn_readfirstlane(pm.first);
+      const int64_t b_off = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(pm.off & 0xffffffff)) |
+                            ((int64_t)__builtin_amdgcn_readfirstlane((int)(pm.off >> 32)) << 32);
+      const int32_t* B = d.pool + b_off;
+      if (dense && b_first < lo) {
         // leave the bit set: write the column back in sorted form
         int32_t W = ((l - lo) >> 5) + 1, total;
         const int32_t per = (W + kLongThreads - 1) / kLongThreads;
@@ -670,11 +719,21 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
       if (dense) {
         // toggle the pivot's rows
         int32_t delta = 0;
-        for (int32_t k = tid; k < b_len; k += kLongThreads) {
-          const int32_t x = B[k] - lo;
-          const uint32_t m = 1u << (x & 31);
-          const uint32_t old = atomicXor(&bits[x >> 5], m);
-          delta += (old & m) ? -1 : 1;
+        constexpr int kU = 4;  // independent loads in flight per thread
+        for (int32_t k0 = tid; k0 < b_len; k0 += kLongThreads * kU) {
+          int32_t x[kU];
+#pragma unroll
+          for (int u = 0; u < kU; u++) {
+            const int32_t k = k0 + u * kLongThreads;
+            x[u] = k < b_len ? B[k] - lo : -1;
+          }
+#pragma unroll
+          for (int u = 0; u < kU; u++)
+            if (x[u] >= 0) {
+              const uint32_t m = 1u << (x[u] & 31);
+              const uint32_t old = atomicXor(&bits[x[u] >> 5], m);
+              delta += (old & m) ? -1 : 1;
+            }
         }
         delta = wave_incl_scan(delta);
         if (lane_id() == kWave - 1) atomicAdd(&sh_count, delta);
@@ -861,11 +920,41 @@ __device__ void heapsort_i32(int32_t* a, int64_t n) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_sort_segments(int64_t nseg, const int64_t* off, int32_t* v) {
+// Short segments (<= kSortShort) are insertion-sorted by one thread each; longer ones are
+// listed for the wavefront sort below.
+constexpr int kSortShort = 16;
+__global__ void k_sort_short(int64_t nseg, const int64_t* off, int32_t* v, int32_t* longs, uint32_t* nlong) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nseg; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = base + threadIdx.x;
+    bool is_long = false;
+    if (s < nseg) {
+      const int64_t b = off[s], len = off[s + 1] - b;
+      if (len > kSortShort) is_long = true;
+      else if (len > 1) {
+        int32_t x[kSortShort];
+        for (int i = 0; i < kSortShort; i++) x[i] = i < len ? v[b + i] : 0x7fffffff;
+        for (int i = 1; i < kSortShort; i++)  // fixed trip counts keep x[] in registers
+          for (int j = i; j > 0; j--) {
+            const int32_t a = x[j - 1], c = x[j];
+            x[j - 1] = min(a, c);
+            x[j] = max(a, c);
+          }
+        for (int i = 0; i < kSortShort; i++)
+          if (i < len) v[b + i] = x[i];
+      }
+    }
+    wave_push(is_long, (int32_t)s, longs, nlong);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_segments(const int32_t* list, const uint32_t* nlist, const int64_t* off,
+                                                       int32_t* v) {
   __shared__ int32_t lds[4][kSortLds];
   const int w = threadIdx.x / kWave, lane = lane_id();
   int32_t* buf = lds[w];
-  for (int64_t s = (int64_t)blockIdx.x * 4 + w; s < nseg; s += (int64_t)gridDim.x * 4) {
+  const int64_t nseg = *nlist;
+  for (int64_t si = (int64_t)blockIdx.x * 4 + w; si < nseg; si += (int64_t)gridDim.x * 4) {
+    const int64_t s = list[si];
     const int64_t b = off[s];
     const int64_t len = off[s + 1] - b;
     if (len <= 1) continue;
@@ -1338,7 +1427,7 @@ class Engine {
   void release() {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
                       &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
-                      &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_})
+                      &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_, &b_long_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
     if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
@@ -1393,7 +1482,7 @@ class Engine {
   std::string err_;
   DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
       b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
-      b_drem_, b_scan_;
+      b_drem_, b_scan_, b_pmeta_, b_prem_, b_long_;
   int64_t arena_hint_ = 0;
   bool timing_ = false;
   int32_t short_max_ = kTile;
@@ -1653,7 +1742,12 @@ int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int
   if (rc) return rc;
   HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
   hipLaunchKernelGGL(k_dual_fill, dim3(g), dim3(256), 0, stream_, n, colptr, rows, d.removed, *dcolptr, cnt, *drows);
-  hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n, 4)), dim3(256), 0, stream_, n, *dcolptr, *drows);
+  int32_t* longs;
+  if ((rc = ensure(b_long_, (size_t)n + 16, &longs))) return rc;
+  uint32_t* nlong = (uint32_t*)(longs + n);
+  HIPCHK(hipMemsetAsync(nlong, 0, sizeof(uint32_t), stream_));
+  hipLaunchKernelGGL(k_sort_short, dim3(g), dim3(256), 0, stream_, n, *dcolptr, *drows, longs, nlong);
+  hipLaunchKernelGGL(k_sort_segments, dim3(cus_ * 8), dim3(256), 0, stream_, longs, nlong, *dcolptr, *drows);
   hipLaunchKernelGGL(k_dual_dims, dim3(g), dim3(256), 0, stream_, n, dims, maxdim, d.removed, *ddims, drem);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(d.removed, drem, (size_t)n, hipMemcpyDeviceToDevice, stream_));
@@ -1691,7 +1785,8 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       (rc = ensure(b_cap_, (size_t)n, &d.cap)) || (rc = ensure(b_low_, (size_t)n, &d.low)) ||
       (rc = ensure(b_state_, (size_t)n, &d.state)) || (rc = ensure(b_removed_, (size_t)n, &d.removed)) ||
       (rc = ensure(b_flag_, (size_t)(2 * n), &d.flag)) || (rc = ensure(b_piv_, (size_t)n, &d.piv)) ||
-      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)cw::kWords, &d.ctr)))
+      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)cw::kWords, &d.ctr)) ||
+      (rc = ensure(b_pmeta_, (size_t)n, &d.pivmeta)))
     return rc;
   int32_t* U;
   if ((rc = ensure(b_U_, (size_t)n, &U))) return rc;
@@ -1732,12 +1827,19 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
   }
   uint8_t* hd = nullptr;
   const bool dual = p.dualize == 1;
+  const int64_t* primal_colptr = nullptr;
+  const int32_t* primal_rows = nullptr;
+  uint8_t* primal_removed = nullptr;
   int64_t nnz_eff = nnz;
   if (dual) {
     // cohomology on the (collapsed) complex: reduce the anti-transpose; guided clearing is a
     // statement about the top dimension of the primal complex and is not used here
     int64_t* dcp;
     int32_t *drw, *ddm;
+    primal_colptr = d_colptr;
+    primal_rows = d_rows;
+    if ((rc = ensure(b_prem_, (size_t)n, &primal_removed))) return rc;
+    HIPCHK(hipMemcpyAsync(primal_removed, d.removed, (size_t)n, hipMemcpyDeviceToDevice, stream_));
     rc = build_dual(d, n, nnz, maxdim, d_colptr, d_rows, d_dims, &dcp, &drw, &ddm);
     if (rc) return rc;
     if (p.model & 2) {  // removed cells drop their entries from the dual
@@ -1764,8 +1866,12 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
     const int g = grid_for(n);
     hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d);
-    hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
-    hipLaunchKernelGGL(k_lookup, dim3(g), dim3(256), 0, stream_, d, (const uint8_t*)hd);
+    if (dual)
+      hipLaunchKernelGGL(k_dual_init_from_primal, dim3(g), dim3(256), 0, stream_, n, primal_colptr, primal_rows,
+                         (const uint8_t*)primal_removed, d.leftone, d.flag + n);
+    else
+      hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
+    hipLaunchKernelGGL(k_lookup, dim3(g), dim3(256), 0, stream_, d, (const uint8_t*)hd, dual ? 1 : 0);
     hipLaunchKernelGGL(k_clearing, dim3(g), dim3(256), 0, stream_, d);
     hipLaunchKernelGGL(k_compress, dim3(g), dim3(256), 0, stream_, d);
     HIPCHK(hipGetLastError());
@@ -1834,6 +1940,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
           for (;;) {
             for (int t = 0; t < rounds_per_sync; t++, r++) {
               hipLaunchKernelGGL(k_claim, dim3(cg), dim3(256), 0, stream_, d, r);
+              hipLaunchKernelGGL(k_pivmeta, dim3(cg), dim3(256), 0, stream_, d, r);
               if (timing_) HIPCHK(hipEventRecord(ev_[4 * t], stream_));
               hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(kWave * kAddWaves), 0, stream_, d, r, fl);
               if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 1], stream_));

@@ -62,7 +62,7 @@ struct ssq_server {
   ssq_server_config cfg;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Request*> queue;
+  std::deque<Request*> queue[2];  // [0] matrices, [1] point clouds
   std::unordered_map<int64_t, Result> done;
   std::vector<std::thread> workers;
   int64_t next_id = 1;
@@ -80,20 +80,27 @@ struct ssq_server {
     }
     for (;;) {
       std::vector<Request*> batch;
+      int batch_kind = 0;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-        if (stop && queue.empty()) break;
+        cv_work.wait(lk, [&] { return stop || !queue[0].empty() || !queue[1].empty(); });
+        if (stop && queue[0].empty() && queue[1].empty()) break;
+        // one kind per batch (each has its own parameters): the queue whose head is older
+        int kind = queue[0].empty() ? 1
+                   : queue[1].empty() ? 0
+                   : (queue[1].front()->t_submit < queue[0].front()->t_submit ? 1 : 0);
+        batch_kind = kind;
+        auto& Q = queue[kind];
         // admission: oldest first while it fits the batch size and the device budget; the
         // first request is always admitted when nothing else is in flight
         int64_t cells = 0;
-        while (!queue.empty() && (int)batch.size() < cfg.max_batch) {
-          Request* q = queue.front();
+        while (!Q.empty() && (int)batch.size() < cfg.max_batch) {
+          Request* q = Q.front();
           if (in_flight_bytes + q->est_bytes > cfg.mem_budget_bytes && (in_flight_bytes > 0 || !batch.empty()))
             break;
           if (cfg.max_batch_cells > 0 && !batch.empty() && cells + q->est_cells > cfg.max_batch_cells) break;
           cells += q->est_cells;
-          queue.pop_front();
+          Q.pop_front();
           in_flight_bytes += q->est_bytes;
           batch.push_back(q);
         }
@@ -108,7 +115,9 @@ struct ssq_server {
       for (size_t i = 0; i < batch.size(); i++) {
         Request* q = batch[i];
         if (q->is_points) {
-          st[i] = ssq::vr_build(q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &mats[i]);
+          // Rips matrices of point requests are built on this worker's GPU stream
+          st[i] = (h && q->maxdim <= 2) ? ssq_vr_build_gpu(h, q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &mats[i])
+                                        : ssq::vr_build(q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &mats[i]);
         } else {
           mats[i].n = (int64_t)q->dims.size();
           mats[i].nnz = (int64_t)q->rows.size();
@@ -127,7 +136,8 @@ struct ssq_server {
       std::vector<ssq_pairs> outs(ptrs.size());
       int brc = SSQ_OK;
       if (!ptrs.empty()) {
-        brc = h ? ssq_batch_reduce(h, ptrs.data(), (int)ptrs.size(), &cfg.params, outs.data(), nullptr) : SSQ_EDEVICE;
+        const ssq_params* bp = batch_kind ? &cfg.points_params : &cfg.params;
+        brc = h ? ssq_batch_reduce(h, ptrs.data(), (int)ptrs.size(), bp, outs.data(), nullptr) : SSQ_EDEVICE;
       }
       const auto t_end = Clock::now();
       {
@@ -171,7 +181,7 @@ struct ssq_server {
       }
       q->id = next_id++;
       q->t_submit = Clock::now();
-      queue.push_back(q);
+      queue[q->is_points ? 1 : 0].push_back(q);
       outstanding++;
       *ticket = q->id;
     }
@@ -266,7 +276,8 @@ extern "C" int ssq_server_destroy(ssq_server* s) {
   s->cv_work.notify_all();
   s->cv_done.notify_all();
   for (auto& t : s->workers) t.join();
-  for (Request* q : s->queue) delete q;
+  for (auto& Q : s->queue)
+    for (Request* q : Q) delete q;
   delete s;
   return SSQ_OK;
 }

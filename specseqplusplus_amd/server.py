@@ -8,11 +8,16 @@ from .api import params as make_params
 
 
 class Server:
-    def __init__(self, device=0, streams=2, max_batch=64, mem_budget_bytes=0, p=None, max_batch_cells=0):
+    def __init__(self, device=0, streams=2, max_batch=64, mem_budget_bytes=0, p=None, max_batch_cells=0,
+                 points_p=None):
+        """p: parameters for submitted matrices; points_p: for point clouds (Rips complexes,
+        cohomology by default)."""
         cfg = L.ssq_server_config()
         cfg.device, cfg.streams, cfg.max_batch, cfg.mem_budget_bytes = device, streams, max_batch, mem_budget_bytes
         cfg.max_batch_cells = max_batch_cells
         cfg.params = p or make_params()
+        cfg.points_params = points_p or make_params(dualize=1, ss_block_size=1 << 30, queue_size=16384, level1=64,
+                                                     level2=1024)
         self._s = ctypes.c_void_p()
         L.check(L.lib().ssq_server_create(ctypes.byref(cfg), ctypes.byref(self._s)), "ssq_server_create")
 

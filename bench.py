@@ -83,6 +83,7 @@ def split_pairs(pairs, base):
     return out
 
 
+BATCH_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=64, level2=1024)
 SERVED_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=16384, level1=64, level2=1024)
 SERVED_MATRIX_PARAMS = dict(dualize=0, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=1000,
                             level2=16000)
@@ -114,6 +115,102 @@ def build_served_pool(seed, n_clouds=48, n_mats=12):
     return pts, mats, (ref_pts, ref_mats)
 
 
+# Column-addition kernels: (name, time field, algorithmic-bytes field).  Algorithmic bytes of
+# one addition a <- a xor b are 4*(|a| + |b| + |a xor b|) + 24 (read both columns, write the
+# result, plus pivot lookups and column metadata); the engine sums them per kernel.
+ADD_KERNELS = [("k_add", "t_add_kernel_ms", "add_bytes"),
+               ("k_add_long", "t_add_long_kernel_ms", "dense_bytes")]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_rips.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (lines 'traffic <kernel> <bytes per launch>'); None when absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                t = line.split()
+                if len(t) == 3 and t[0] == "traffic" and t[1] == kernel:
+                    return float(t[2])
+    except OSError:
+        pass
+    return None
+
+
+def roofline_record(stats):
+    """Roofline of the column-addition kernel with the largest measured time: algorithmic bytes
+    per launch over its average launch duration (HIP events on the engine's stream)."""
+    launches = sum(s["n_add_launches"] for s in stats)  # k_add and k_add_long launch once per round each
+    per = []
+    for name, tf, bf in ADD_KERNELS:
+        per.append((sum(s[tf] for s in stats), sum(s[bf] for s in stats), name))
+    ms, nbytes, name = max(per)
+    achieved = (nbytes / 1e9) / (ms / 1e3) if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(name),
+            "kernel": name, "launches": int(launches), "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
+            "bytes_per_launch": round(nbytes / max(launches, 1), 1),
+            "kernels_ms": {n: round(t, 3) for t, _, n in per}}
+
+
+def single_config(args):
+    """One single-complex configuration: the GPU reduction with the matrix resident in HBM,
+    the oracle on one host core, and B_alg / t_gpu, where B_alg = the oracle's addition bytes
+    (same sub-complex) + 4*nnz + 12*N for the input pass + 16*#pairs for the output."""
+    import specseqplusplus_amd as S
+    from specseqplusplus_amd import configs as CF
+    from oracle import oracle as O
+    O.build()
+    t0 = time.time()
+    if args.single == "c1":
+        m, kw, what = S.vr_build(CF.c1_points(), 0.25, 2), dict(CF.C1_PARAMS), "C1: Rips of 512 uniform points, r=0.25, dim<=2"
+    elif args.single == "c2":
+        m, kw = S.vr_build(CF.c2_points(), 1.0, 2), dict(CF.C2_PARAMS)
+        what = "C2: Rips of 4096 Haar rotations in R^9, r=1.0, dim<=2, model 3"
+    else:
+        from specseqplusplus_amd.alpha import alpha_complex, c3_points
+        m, kw = alpha_complex(c3_points(), 90.0), dict(model=0, queue_size=20000, level1=1000, level2=16000)
+        what = "C3: alpha complex of 1e5 uniform points in a 100-cube, alpha^2<=90, dim<=3"
+    print(f"[bench] {what}: {m.n} cells, {m.nnz} nonzeros (built in {time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    h = S.Handle(0)
+    dm = h.upload(m)
+    out = h.device_empty(2 * m.n, np.int64)
+    p = S.params(**kw)
+    for _ in range(args.warmup):
+        h.reduce_resident(dm, out, p)
+    h.synchronize()
+    sts = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        sts.append(h.reduce_resident(dm, out, p))
+    h.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / args.steps
+    n_pairs, st = sts[-1]
+    pairs = out.numpy(2 * n_pairs).reshape(-1, 2)
+    mo = O.Matrix(m.colptr, m.rows, m.dims)
+    skip = O.collapse_skip(mo) if kw.get("model", 0) & 2 else None
+    tt = time.perf_counter()
+    ref, ost = O.twist_pairs(mo, skip=skip, stats=True)
+    cpu_s = time.perf_counter() - tt
+    b_alg = ost["bytes"] + 4.0 * m.nnz + 12.0 * m.n + 16.0 * len(ref)
+    achieved = b_alg / (ms / 1e3) / 1e9
+    rec = {"metric": f"single-complex reduction time ({args.single})", "value": round(ms, 3), "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "dtype": "int32", "data": "synthetic (seeded)",
+           "config": {"workload": what, "cells": int(m.n), "nnz": int(m.nnz), "params": kw},
+           "bit_exact": bool(np.array_equal(pairs, ref)), "pairs": int(len(ref)),
+           "engine": {k: st[k] for k in ("t_init_ms", "t_collapse_ms", "t_reduce_ms", "n_additions", "n_apparent",
+                                         "n_cleared_hd", "n_kept_edges", "n_pages", "n_rounds", "max_chain")},
+           "b_alg_roofline": {"b_alg_bytes": b_alg, "oracle_additions": ost["additions"], "achieved": round(achieved, 2),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6)},
+           "cpu_baseline": {"value": round(cpu_s * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                            "sample": "the whole matrix, oracle twist reduction (bit-tree pivot column, clearing)"
+                                      + (", on the GUDHI-collapsed sub-complex" if skip is not None else "")}}
+    print(json.dumps(rec))
+    h.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,11 +227,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--single", choices=["c1", "c2", "c3"],
+                    help="time one single-complex configuration instead of the batched stream")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default batched parameters instead of tuning each size class")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every complex is independent, so more GPUs would each run "
               "their own copy of this stream; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.single:
+        single_config(args)
+        return
 
     import specseqplusplus_amd as S
     from specseqplusplus_amd.server import Server
@@ -154,8 +259,10 @@ def main():
     classes = [c for c in classes if len(c)]
     plans = []
     for ci, idx in enumerate(classes):
-        sample = [mats[i] for i in idx[:min(len(idx), 96)]]
-        chosen, table = tune(h, sample)
+        if args.no_tune:
+            chosen, table = dict(BATCH_PARAMS), []
+        else:
+            chosen, table = tune(h, [mats[i] for i in idx[:min(len(idx), 96)]])
         bd, cbase = bdiag([mats[i] for i in idx])
         plans.append({"idx": idx, "dm": h.upload(bd), "out": h.device_empty(2 * bd.n, np.int64),
                       "p": S.params(**chosen), "chosen": chosen, "table": table, "base": cbase,
@@ -196,26 +303,48 @@ def main():
     parity = {"checked": 0, "bit_exact": True}
     cpu = None
     if not args.no_cpu:
+        from concurrent.futures import FIRST_COMPLETED, wait
         from oracle import oracle as O
         O.build()
-        order = np.random.default_rng(1).permutation(args.complexes)
-        tcpu, done = 0.0, 0
-        for i in order:
+        # one complex per core (the oracle is re-entrant and ctypes drops the GIL during the
+        # call), random order, until the time budget is spent
+        workers = max(1, min(16, os.cpu_count() or 1))
+        order = iter(np.random.default_rng(1).permutation(args.complexes))
+
+        def work(i):
             m = mats[i]
-            mo = O.Matrix(m.colptr, m.rows, m.dims)
             tt = time.perf_counter()
-            ref = O.twist_pairs(mo)
-            tcpu += time.perf_counter() - tt
-            done += 1
-            ok = ref.shape == per[i].shape and np.array_equal(ref, per[i])
-            parity["bit_exact"] &= bool(ok)
-            if tcpu > args.cpu_seconds:
-                break
+            ref = O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims))
+            return i, ref, time.perf_counter() - tt
+
+        done, busy = 0, 0.0
+        t_start = time.perf_counter()
+        with ThreadPoolExecutor(workers) as ex:
+            futs = set()
+            while True:
+                while len(futs) < 2 * workers and time.perf_counter() - t_start < args.cpu_seconds:
+                    i = next(order, None)
+                    if i is None:
+                        break
+                    futs.add(ex.submit(work, i))
+                if not futs:
+                    break
+                fin, futs = wait(futs, return_when=FIRST_COMPLETED)
+                for f in fin:
+                    i, ref, dt = f.result()
+                    busy += dt
+                    done += 1
+                    ok = ref.shape == per[i].shape and np.array_equal(ref, per[i])
+                    parity["bit_exact"] &= bool(ok)
+        wall = time.perf_counter() - t_start
         parity["checked"] = done
-        cpu = {"value": done / tcpu, "unit": "diagrams/s", "cores": 1, "kind": "port",
+        cpu = {"value": done / wall, "unit": "diagrams/s", "cores": workers, "kind": "port",
+               "single_core_value": done / busy,
                "sample": f"{done} complexes of this stream (random order), oracle twist reduction "
-                         f"(bit-tree pivot column, clearing), single thread, {tcpu:.1f}s"}
-        print(f"[bench] parity {parity}; cpu {cpu['value']:.1f} diagrams/s", file=sys.stderr, flush=True)
+                         f"(bit-tree pivot column, clearing), one complex per thread on {workers} threads, "
+                         f"{wall:.1f}s wall"}
+        print(f"[bench] parity {parity}; cpu {cpu['value']:.1f} diagrams/s on {workers} threads "
+              f"({cpu['single_core_value']:.1f}/s per core)", file=sys.stderr, flush=True)
 
     # plain per-complex GPU: one engine call per complex, matrices already resident
     plain = None
@@ -240,6 +369,24 @@ def main():
         pool_pts, pool_mats, pool_ref = build_served_pool(args.seed + 7)
         print(f"[bench] served pool: {len(pool_pts)} clouds, {len(pool_mats)} matrices "
               f"({min(m.n for m in pool_mats)}-{max(m.n for m in pool_mats)} cells)", file=sys.stderr, flush=True)
+        # plain per-request GPU: the same request mix, one request at a time (Rips build on the
+        # GPU for clouds, then a single-complex reduction), host data in and pairs out
+        rng = np.random.default_rng(99)
+        n_plain = max(1, min(args.served, 400))
+        kinds = rng.integers(0, 2, n_plain)
+        picks = rng.integers(0, 1 << 30, n_plain)
+        pp_pts, pp_mat = S.params(**SERVED_PARAMS), S.params(**SERVED_MATRIX_PARAMS)
+        h.reduce(pool_mats[0], pp_mat)
+        tt = time.perf_counter()
+        for k in range(n_plain):
+            if kinds[k] == 0:
+                X, r = pool_pts[picks[k] % len(pool_pts)]
+                h.reduce(h.vr_build_gpu(X, r, 2), pp_pts)
+            else:
+                h.reduce(pool_mats[picks[k] % len(pool_mats)], pp_mat)
+        served["plain_per_request_diagrams_per_s"] = round(n_plain / (time.perf_counter() - tt), 1)
+        print(f"[bench] served mix, plain one request at a time: "
+              f"{served['plain_per_request_diagrams_per_s']} diagrams/s", file=sys.stderr, flush=True)
         for qps in args.qps:
             rng = np.random.default_rng(int(qps))
             kinds = rng.integers(0, 2, args.served)
@@ -271,22 +418,16 @@ def main():
                     lat.append(res[2])
                     exact &= bool(np.array_equal(res[0], pool_ref[kind][j]))
             lat = np.array(lat)
-            rec = {"qps_offered": qps, "diagrams_per_s": round(args.served / (te - ts), 1),
+            rate = args.served / (te - ts)
+            rec = {"qps_offered": qps, "diagrams_per_s": round(rate, 1),
+                   "over_plain": round(rate / served["plain_per_request_diagrams_per_s"], 2),
                    "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                    "bit_exact": exact}
             served["loads"].append(rec)
             print(f"[bench] served {rec}", file=sys.stderr, flush=True)
 
     st = {k: sum(x[k] for x in stats[-len(plans):]) for k in stats[-1] if isinstance(stats[-1][k], (int, float))}
-    add_ms = sum(s["t_add_kernel_ms"] for s in stats)
-    add_bytes = sum(s["add_bytes"] for s in stats)
-    launches = sum(s["n_add_launches"] for s in stats)
-    achieved = (add_bytes / 1e9) / (add_ms / 1e3) if add_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_add (column additions)", "launches": int(launches),
-                "avg_launch_us": round(add_ms * 1e3 / max(launches, 1), 2),
-                "bytes_per_launch": round(add_bytes / max(launches, 1), 1)}
+    roofline = roofline_record(stats)
     out = {
         "metric": METRIC, "value": round(diagrams_per_s, 2), "unit": "diagrams/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
@@ -295,7 +436,7 @@ def main():
         "config": {"workload": f"batched: {args.complexes} Rips complexes, n log-uniform [{args.nmin},{args.nmax}], "
                                f"cube/circle/torus, r = 30th percentile, dim<=2, block-diagonal in HBM",
                    "cells": int(N), "nnz": int(NNZ),
-                   "size_classes": [{"complexes": int(len(pl["idx"])), "cells": int(pl["cells"]),
+                   "size_classes": [{"complexes": int(len(pl["idx"])), "cells": int(pl["cells"]), "nnz": int(pl["nnz"]),
                                      "params": pl["chosen"], "tuning_ms": [t["ms"] for t in pl["table"]]}
                                     for pl in plans]},
         "plain_gpu_diagrams_per_s": round(plain, 2) if plain else None,

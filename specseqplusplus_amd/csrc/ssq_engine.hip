@@ -46,11 +46,22 @@ constexpr int kMaxDims = 8;            // cell dimensions 0..7
 
 // Where the pivot column of a row is stored, written once the pivot is established so that
 // an addition can fetch the pivot's id and its storage in one round trip.
-struct __align__(16) PivMeta {
-  int64_t off;    // start of the pivot column in the column store
-  int32_t len;    // its length
-  int32_t first;  // its smallest row
+struct __align__(32) PivMeta {
+  int64_t off;     // start of the pivot column in the column store
+  int32_t len;     // its length
+  int32_t first;   // its smallest row
+  int32_t second;  // its second-largest row (the largest is the row itself), -1 if none
+  int32_t pad[3];
 };
+__device__ __forceinline__ PivMeta pivmeta_of(int64_t off, int32_t len, const int32_t* col) {
+  PivMeta m;
+  m.off = off;
+  m.len = len;
+  m.first = col[0];
+  m.second = len >= 2 ? col[len - 2] : -1;
+  m.pad[0] = m.pad[1] = m.pad[2] = 0;
+  return m;
+}
 
 struct Dev {
   int64_t n;
@@ -98,8 +109,28 @@ constexpr int kChain = 60;     // [60] longest chain, [61] largest |a|+|b|
 constexpr int kWords = 64;
 }  // namespace cw
 
+// Element loops over cells.  Workgroups are dispatched round-robin over the 8 XCDs, so with a
+// grid that is a multiple of 8, workgroup b runs on XCD b % 8.  Each XCD is given one
+// contiguous eighth of [0, n): the cells in flight on an XCD at a time are neighbours, and
+// their scattered accesses (rows of a column, cofaces, pivot tables) stay within the few
+// complexes of a block-diagonal batch that that XCD's L2 holds.  Other grids stride plainly.
+constexpr int kXcds = 8;
+__device__ __forceinline__ bool xcd_grid() { return gridDim.x >= kXcds && (gridDim.x % kXcds) == 0; }
+__device__ __forceinline__ int64_t xcd_span(int64_t n) { return (n + kXcds - 1) / kXcds; }
+__device__ __forceinline__ int64_t xs_begin(int64_t n) {
+  if (!xcd_grid()) return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  return (int64_t)(blockIdx.x % kXcds) * xcd_span(n) + (int64_t)(blockIdx.x / kXcds) * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ int64_t xs_step() {
+  return xcd_grid() ? (int64_t)(gridDim.x / kXcds) * blockDim.x : (int64_t)gridDim.x * blockDim.x;
+}
+__device__ __forceinline__ int64_t xs_end(int64_t n) {
+  if (!xcd_grid()) return n;
+  const int64_t e = (int64_t)(blockIdx.x % kXcds + 1) * xcd_span(n);
+  return e < n ? e : n;
+}
 #define GRID_STRIDE(i, n) \
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t i = xs_begin(n), i##_step = xs_step(), i##_end = xs_end(n); i < i##_end; i += i##_step)
 
 // ------------------------------------------------------------------------------------
 // allocation in the column store (level slots, then the exact-size overflow arena)
@@ -179,6 +210,7 @@ __global__ void k_leftone(Dev d) {
 // in its low row is an apparent pair; a column leftmost in any row is negative
 // (flag[n + col] = 1).  The two flags live in separate bytes: plain stores of 1 only.
 __global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
+  uint32_t apparent = 0;  // summed per wavefront: one atomic per wave, not per pair
   GRID_STRIDE(c, d.n) {
     const int32_t l = d.len[c];
     if (l == 0) continue;
@@ -187,7 +219,7 @@ __global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
     if (d.leftone[lw] == (uint32_t)c) {
       d.piv[lw] = (uint32_t)c;
       d.state[c] = ST_PIVOT;
-      atomicAdd(&d.ctr[cw::kApparent], 1u);
+      apparent++;
     }
     if (neg_known) continue;
     const int32_t* col = d.pool + d.off[c];
@@ -197,6 +229,8 @@ __global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
         break;
       }
   }
+  for (int k = kWave / 2; k > 0; k >>= 1) apparent += __shfl_xor(apparent, k, kWave);
+  if (lane_id() == 0 && apparent) atomicAdd(&d.ctr[cw::kApparent], apparent);
   if (hdclear)
     GRID_STRIDE(c, d.n) if (hdclear[c]) d.flag[c] = 1;
 }
@@ -241,7 +275,7 @@ __global__ void k_compress(Dev d) {
     d.len[c] = o;
     d.low[c] = o ? col[o - 1] : -1;
     if (!o) d.state[c] = ST_ZERO;
-    else if (d.state[c] == ST_PIVOT) d.pivmeta[col[o - 1]] = PivMeta{d.off[c], o, col[0]};
+    else if (d.state[c] == ST_PIVOT) d.pivmeta[col[o - 1]] = pivmeta_of(d.off[c], o, col);
   }
 }
 
@@ -255,7 +289,7 @@ __global__ void k_pivmeta(Dev d, int r) {
     const int32_t l = d.low[c];
     if (d.piv[l] == (uint32_t)c) {
       const int64_t o = d.off[c];
-      d.pivmeta[l] = PivMeta{o, d.len[c], d.pool[o]};
+      d.pivmeta[l] = pivmeta_of(o, d.len[c], d.pool + o);
     }
   }
 }
@@ -320,79 +354,113 @@ __global__ void k_dimstats(Dev d, int32_t* maxrow, int32_t* minrow) {
 // ordered compaction by a small key (stable partition), used for the per-dimension
 // pending lists and for the pair list
 // ------------------------------------------------------------------------------------
-constexpr int kCompactBlock = 256;
+constexpr int kCompactBlock = 256;                   // threads per compaction workgroup
+constexpr int kCompactSub = 4;                       // consecutive sub-tiles per workgroup
+constexpr int kCompactItems = kCompactBlock * kCompactSub;
 
+// Per workgroup and key: how many of its kCompactItems cells have that key.
 template <typename KeyF>
-__global__ void k_key_count(int64_t n, KeyF key, uint32_t* counts, int nkeys) {
+__global__ void __launch_bounds__(kCompactBlock) k_key_count(int64_t n, KeyF key, uint32_t* counts, int nkeys) {
   __shared__ uint32_t s[kMaxDims];
   if (threadIdx.x < (unsigned)nkeys) s[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
-  if (i < n) {
-    const int k = key(i);
-    if (k >= 0) atomicAdd(&s[k], 1u);
+  uint32_t mine[kMaxDims] = {};  // lane 0 of each wave accumulates its wave's counts
+  for (int j = 0; j < kCompactSub; j++) {
+    const int64_t i = (int64_t)blockIdx.x * kCompactItems + j * kCompactBlock + threadIdx.x;
+    const int k = i < n ? key(i) : -1;
+#pragma unroll
+    for (int kk = 0; kk < kMaxDims; kk++)  // static indexing keeps mine[] in registers
+      if (kk < nkeys) mine[kk] += (uint32_t)__popcll(__ballot(k == kk));
+  }
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int kk = 0; kk < kMaxDims; kk++)
+      if (kk < nkeys && mine[kk]) atomicAdd(&s[kk], mine[kk]);
   }
   __syncthreads();
   if (threadIdx.x < (unsigned)nkeys) counts[(int64_t)blockIdx.x * nkeys + threadIdx.x] = s[threadIdx.x];
 }
 
-// exclusive scan over blocks, keys laid out key-major in the output: base of key k is the
-// total of all smaller keys.  One workgroup; totals[k] receives each key's count.
-__global__ void k_key_scan(int64_t nblocks, uint32_t* counts, int nkeys, int64_t* totals) {
+__device__ __forceinline__ int64_t wave_incl_scan64(int64_t v) {
+  const int lane = lane_id();
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int64_t o = __shfl_up(v, d, kWave);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Exclusive scan over workgroups, keys laid out key-major in the output: the base of key k
+// is the total of all smaller keys.  One workgroup of kKeyScanThreads; totals[k] receives
+// each key's count.
+constexpr int kKeyScanThreads = 1024;
+__global__ void __launch_bounds__(kKeyScanThreads) k_key_scan(int64_t nblocks, uint32_t* counts, int nkeys,
+                                                             int64_t* totals) {
+  __shared__ int64_t s_w[kKeyScanThreads / kWave];
   __shared__ int64_t s_tot[kMaxDims];
-  __shared__ int64_t s_part[1024];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int64_t per = (nblocks + kKeyScanThreads - 1) / kKeyScanThreads;
+  const int64_t b0 = min((int64_t)threadIdx.x * per, nblocks);
+  const int64_t b1 = min(b0 + per, nblocks);
   int64_t key_base = 0;
   for (int k = 0; k < nkeys; k++) {
-    // per-thread sequential chunk, then block scan of chunk sums
-    const int64_t per = (nblocks + blockDim.x - 1) / blockDim.x;
-    const int64_t b0 = (int64_t)threadIdx.x * per;
-    const int64_t b1 = b0 + per < nblocks ? b0 + per : nblocks;
     int64_t sum = 0;
     for (int64_t b = b0; b < b1; b++) sum += counts[b * nkeys + k];
-    s_part[threadIdx.x] = sum;
+    const int64_t incl = wave_incl_scan64(sum);
+    if (lane == kWave - 1) s_w[w] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t run = 0;
-      for (unsigned t = 0; t < blockDim.x; t++) {
-        const int64_t v = s_part[t];
-        s_part[t] = run;
-        run += v;
-      }
-      s_tot[k] = run;
+    int64_t before = 0, tot = 0;
+    for (int ww = 0; ww < kKeyScanThreads / kWave; ww++) {
+      const int64_t v = s_w[ww];
+      if (ww < w) before += v;
+      tot += v;
     }
-    __syncthreads();
-    int64_t run = key_base + s_part[threadIdx.x];
+    int64_t run = key_base + before + incl - sum;
     for (int64_t b = b0; b < b1; b++) {
       const uint32_t v = counts[b * nkeys + k];
       counts[b * nkeys + k] = (uint32_t)run;  // base offsets fit: n < 2^31
       run += v;
     }
-    key_base += s_tot[k];
-    __syncthreads();
+    if (threadIdx.x == 0) s_tot[k] = tot;
+    key_base += tot;
+    __syncthreads();  // s_w is reused by the next key
   }
   if (threadIdx.x == 0)
     for (int k = 0; k < nkeys; k++) totals[k] = s_tot[k];
 }
 
+// Stable scatter: cells keep their order within each key (sub-tiles, then waves, then lanes).
 template <typename KeyF, typename EmitF>
-__global__ void k_key_scatter(int64_t n, KeyF key, const uint32_t* bases, int nkeys, EmitF emit) {
+__global__ void __launch_bounds__(kCompactBlock) k_key_scatter(int64_t n, KeyF key, const uint32_t* bases, int nkeys,
+                                                              EmitF emit) {
   __shared__ uint32_t s_wave[kCompactBlock / kWave][kMaxDims];
-  const int64_t i = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
-  const int k = i < n ? key(i) : -1;
+  __shared__ uint32_t s_run[kMaxDims];
   const int lane = lane_id();
   const int w = threadIdx.x / kWave;
   const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  uint32_t rank_in_wave = 0;
-  for (int kk = 0; kk < nkeys; kk++) {
-    const uint64_t m = __ballot(k == kk);
-    if (k == kk) rank_in_wave = (uint32_t)__popcll(m & lower);
-    if (lane == 0) s_wave[w][kk] = (uint32_t)__popcll(m);
-  }
-  __syncthreads();
-  if (k >= 0) {
-    uint32_t before = 0;
-    for (int ww = 0; ww < w; ww++) before += s_wave[ww][k];
-    emit(i, (int64_t)bases[(int64_t)blockIdx.x * nkeys + k] + before + rank_in_wave);
+  if (threadIdx.x < (unsigned)nkeys) s_run[threadIdx.x] = bases[(int64_t)blockIdx.x * nkeys + threadIdx.x];
+  for (int j = 0; j < kCompactSub; j++) {
+    const int64_t i = (int64_t)blockIdx.x * kCompactItems + j * kCompactBlock + threadIdx.x;
+    const int k = i < n ? key(i) : -1;
+    uint32_t rank_in_wave = 0;
+    for (int kk = 0; kk < nkeys; kk++) {
+      const uint64_t m = __ballot(k == kk);
+      if (k == kk) rank_in_wave = (uint32_t)__popcll(m & lower);
+      if (lane == 0) s_wave[w][kk] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (k >= 0) {
+      uint32_t before = s_run[k];
+      for (int ww = 0; ww < w; ww++) before += s_wave[ww][k];
+      emit(i, (int64_t)before + rank_in_wave);
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)nkeys) {
+      uint32_t t = 0;
+      for (int ww = 0; ww < kCompactBlock / kWave; ww++) t += s_wave[ww][threadIdx.x];
+      s_run[threadIdx.x] += t;
+    }
+    __syncthreads();
   }
 }
 
@@ -502,12 +570,35 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
     int32_t* sO = bufO;
     uint32_t chain = 0;
     bool deferred = false;
+    // a2: the running column's largest row below l (-1 none).  When it differs from the
+    // pivot's, the larger of the two is the next low: that low's pivot is fetched together
+    // with the current pivot's rows.
+    int32_t a2 = a_len >= 2 ? d.pool[a_off + a_len - 2] : -1;
+    int32_t spec_row = -2;
+    uint32_t spec_p = kEmpty;
+    PivMeta spec_pm{};
     for (;;) {
-      const uint32_t p = d.piv[l];
-      const PivMeta pm = d.pivmeta[l];
+      uint32_t p;
+      PivMeta pm;
+      if (l == spec_row) {
+        p = spec_p;
+        pm = spec_pm;
+      } else {
+        p = d.piv[l];
+        pm = d.pivmeta[l];
+      }
       if (p == kEmpty || p >= (uint32_t)c) break;
       const int32_t b_len = pm.len;
       const int32_t* B = d.pool + pm.off;
+      spec_row = -2;
+      if (a2 != pm.second) {
+        const int32_t cand = max(a2, pm.second);
+        if (cand >= 0 && cand >= minrow) {
+          spec_row = cand;
+          spec_p = d.piv[cand];
+          spec_pm = d.pivmeta[cand];
+        }
+      }
       int32_t r_len;
       if (a_len + b_len <= d.short_max) {
         if (!in_lds) {
@@ -541,7 +632,8 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
         l = -1;
         break;
       }
-      l = in_lds ? sA[a_len - 1] : d.pool[a_off + a_len - 1];
+      l = sA[a_len - 1];
+      a2 = a_len >= 2 ? sA[a_len - 2] : -1;
       if (l < minrow) break;
     }
     max_chain = max(max_chain, chain);
@@ -618,6 +710,28 @@ __device__ __forceinline__ int32_t block_max_i32(int32_t v, int32_t* scratch) {
   return m;
 }
 
+// Top two of distinct values (-1 = none): (a1 > a2) merged with (b1 > b2).
+__device__ __forceinline__ void top2_merge(int32_t& a1, int32_t& a2, int32_t b1, int32_t b2) {
+  a2 = max(min(a1, b1), max(a2, b2));
+  a1 = max(a1, b1);
+}
+__device__ __forceinline__ void block_top2(int32_t& v1, int32_t& v2, int32_t* scratch) {
+  for (int k = kWave / 2; k > 0; k >>= 1) {
+    const int32_t o1 = __shfl_xor(v1, k, kWave), o2 = __shfl_xor(v2, k, kWave);
+    top2_merge(v1, v2, o1, o2);
+  }
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane_id() == 0) {
+    scratch[2 * w] = v1;
+    scratch[2 * w + 1] = v2;
+  }
+  __syncthreads();
+  v1 = scratch[0];
+  v2 = scratch[1];
+  for (int k = 1; k < kLongThreads / kWave; k++) top2_merge(v1, v2, scratch[2 * k], scratch[2 * k + 1]);
+}
+
 __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* scratch, int32_t* total) {
   const int w = threadIdx.x / kWave;
   const int incl = wave_incl_scan(v);
@@ -636,7 +750,7 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* scratch, 
 __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t minrow) {
   __shared__ uint32_t bits[kLongWords];
   __shared__ int32_t stage[3][kTileBuf];
-  __shared__ int32_t red[kLongThreads / kWave];
+  __shared__ int32_t red[2 * (kLongThreads / kWave)];
   __shared__ int32_t sh_count, sh_flag;
   __shared__ int64_t sh_off;
   __shared__ int32_t sh_cap;
@@ -657,6 +771,14 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
     bool dense = d.pool[a_off] >= lo;
     bool failed = false;
     uint32_t chain = 0;
+    // a2: the running column's largest row below l (-1 none, kUnknown not known).  When it
+    // differs from the pivot's, the larger of the two is the next low, so that low's pivot
+    // is fetched while the current addition is still toggling rows.
+    constexpr int32_t kUnknown = -3;
+    int32_t a2 = a_len >= 2 ? d.pool[a_off + a_len - 2] : -1;
+    int32_t spec_row = -2;
+    uint32_t spec_p = kEmpty;
+    PivMeta spec_pm{};
     if (dense) {
       const int32_t nw = ((l - lo) >> 5) + 1;
       for (int32_t w = tid; w < nw; w += kLongThreads) bits[w] = 0u;
@@ -672,11 +794,20 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
     for (;;) {
       // the chase state is uniform across the workgroup: scalar loads
       l = __builtin_amdgcn_readfirstlane(l);
-      const uint32_t p = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.piv[l]);
-      const PivMeta pm = d.pivmeta[l];
+      uint32_t p;
+      PivMeta pm;
+      if (l == spec_row) {
+        p = spec_p;
+        pm = spec_pm;
+      } else {
+        p = d.piv[l];
+        pm = d.pivmeta[l];
+      }
+      p = (uint32_t)__builtin_amdgcn_readfirstlane((int)p);
       if (p == kEmpty || p >= (uint32_t)c) break;
       const int32_t b_len = __builtin_amdgcn_readfirstlane(pm.len);
       const int32_t b_first = __builtin_amdgcn_readfirstlane(pm.first);
+      const int32_t b2 = __builtin_amdgcn_readfirstlane(pm.second);
       const int64_t b_off = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(pm.off & 0xffffffff)) |
                             ((int64_t)__builtin_amdgcn_readfirstlane((int)(pm.off >> 32)) << 32);
       const int32_t* B = d.pool + b_off;
@@ -717,9 +848,18 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
       }
       int32_t r_len;
       if (dense) {
+        spec_row = -2;
+        if (a2 != kUnknown && a2 != b2) {
+          const int32_t cand = max(a2, b2);
+          if (cand >= 0 && cand >= minrow) {
+            spec_row = cand;
+            spec_p = d.piv[cand];
+            spec_pm = d.pivmeta[cand];
+          }
+        }
         // toggle the pivot's rows
         int32_t delta = 0;
-        constexpr int kU = 4;  // independent loads in flight per thread
+        constexpr int kU = 16;  // independent loads in flight per thread
         for (int32_t k0 = tid; k0 < b_len; k0 += kLongThreads * kU) {
           int32_t x[kU];
 #pragma unroll
@@ -739,14 +879,34 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
         if (lane_id() == kWave - 1) atomicAdd(&sh_count, delta);
         __syncthreads();
         r_len = sh_count;
-        // new low: highest set bit, scanning down from the old low's word
-        int32_t found = -1;
-        for (int32_t base = (l - lo) >> 5; base >= 0 && found < 0; base -= kLongThreads) {
+        // new low and the row below it: the two highest set bits, scanning down from the old
+        // low's word; the second is looked for at most one range past the first
+        int32_t t1 = -1, t2 = -1, ranges_past = 0;
+        int32_t base = r_len > 0 ? (l - lo) >> 5 : -1;  // r_len is uniform (read from LDS)
+        for (; base >= 0; base -= kLongThreads) {  // t1, t2 are uniform: the exits are too
           const int32_t w = base - tid;
-          const int32_t cand = (w >= 0 && bits[w]) ? w : -1;
-          found = block_max_i32(cand, red);
+          int32_t x1 = -1, x2 = -1;
+          if (w >= 0) {
+            uint32_t m = bits[w];
+            if (m) {
+              const int h = 31 - __clz(m);
+              x1 = lo + (w << 5) + h;
+              m &= ~(1u << h);
+              if (m) x2 = lo + (w << 5) + (31 - __clz(m));
+            }
+          }
+          block_top2(x1, x2, red);
+          if (t1 < 0) {
+            t1 = x1;
+            t2 = x2;
+          } else {
+            t2 = x1;
+          }
+          if (t2 >= 0 || (t1 >= 0 && r_len == 1)) break;
+          if (t1 >= 0 && ++ranges_past >= 2) break;
         }
-        const int32_t newl = found < 0 ? -1 : lo + (found << 5) + (31 - __clz(bits[found]));
+        const int32_t newl = t1;
+        a2 = t2 >= 0 ? t2 : ((base < 0 || r_len <= 1) ? -1 : kUnknown);
         n_dense += 1;
         n_dbytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
         max_len = max(max_len, (uint32_t)(a_len + b_len));
@@ -895,8 +1055,9 @@ __global__ void k_dual_dims(int64_t n, const int32_t* dims, int32_t maxd, const 
   }
 }
 
-// Sort every segment v[off[s] : off[s+1]) ascending: one wavefront per segment; bitonic in
-// registers up to 64 entries, in LDS up to kSortLds, heapsort by one lane beyond (rare).
+// Sort every segment v[off[s] : off[s+1]) ascending.  Segments longer than a lane's share
+// are sorted by a wavefront: bitonic in registers up to 64 entries, in LDS up to kSortLds,
+// heapsort by one lane beyond (rare).
 constexpr int kSortLds = 2048;
 __device__ void heapsort_i32(int32_t* a, int64_t n) {
   auto sift = [&](int64_t i, int64_t m) {
@@ -920,19 +1081,23 @@ __device__ void heapsort_i32(int32_t* a, int64_t n) {
   }
 }
 
-// Short segments (<= kSortShort) are insertion-sorted by one thread each; longer ones are
-// listed for the wavefront sort below.
+// A wavefront takes 64 consecutive segments: each lane sorts its own segment in registers
+// when it has at most kSortShort entries (fixed compare-exchange network); the longer
+// segments of the group are then sorted one after another by the whole wavefront.
 constexpr int kSortShort = 16;
-__global__ void k_sort_short(int64_t nseg, const int64_t* off, int32_t* v, int32_t* longs, uint32_t* nlong) {
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nseg; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = base + threadIdx.x;
-    bool is_long = false;
-    if (s < nseg) {
-      const int64_t b = off[s], len = off[s + 1] - b;
-      if (len > kSortShort) is_long = true;
-      else if (len > 1) {
+__global__ void __launch_bounds__(256) k_sort_segments(int64_t nseg, const int64_t* off, int32_t* v) {
+  __shared__ int32_t lds[4][kSortLds];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  int32_t* buf = lds[w];
+  for (int64_t g0 = ((int64_t)blockIdx.x * 4 + w) * kWave; g0 < nseg; g0 += (int64_t)gridDim.x * 4 * kWave) {
+    const int64_t s0 = g0 + lane;
+    int64_t mylen = 0;
+    if (s0 < nseg) {
+      const int64_t b = off[s0];
+      mylen = off[s0 + 1] - b;
+      if (mylen > 1 && mylen <= kSortShort) {
         int32_t x[kSortShort];
-        for (int i = 0; i < kSortShort; i++) x[i] = i < len ? v[b + i] : 0x7fffffff;
+        for (int i = 0; i < kSortShort; i++) x[i] = i < mylen ? v[b + i] : 0x7fffffff;
         for (int i = 1; i < kSortShort; i++)  // fixed trip counts keep x[] in registers
           for (int j = i; j > 0; j--) {
             const int32_t a = x[j - 1], c = x[j];
@@ -940,57 +1105,50 @@ __global__ void k_sort_short(int64_t nseg, const int64_t* off, int32_t* v, int32
             x[j] = max(a, c);
           }
         for (int i = 0; i < kSortShort; i++)
-          if (i < len) v[b + i] = x[i];
+          if (i < mylen) v[b + i] = x[i];
       }
     }
-    wave_push(is_long, (int32_t)s, longs, nlong);
-  }
-}
-
-__global__ void __launch_bounds__(256) k_sort_segments(const int32_t* list, const uint32_t* nlist, const int64_t* off,
-                                                       int32_t* v) {
-  __shared__ int32_t lds[4][kSortLds];
-  const int w = threadIdx.x / kWave, lane = lane_id();
-  int32_t* buf = lds[w];
-  const int64_t nseg = *nlist;
-  for (int64_t si = (int64_t)blockIdx.x * 4 + w; si < nseg; si += (int64_t)gridDim.x * 4) {
-    const int64_t s = list[si];
-    const int64_t b = off[s];
-    const int64_t len = off[s + 1] - b;
-    if (len <= 1) continue;
-    if (len <= kWave) {
-      int32_t x = lane < len ? v[b + lane] : 0x7fffffff;
-      for (int k = 2; k <= kWave; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-          const int32_t y = __shfl_xor(x, j, kWave);
-          const bool up = (lane & k) == 0, lower = (lane & j) == 0;
-          x = (lower == up) ? min(x, y) : max(x, y);
-        }
-      if (lane < len) v[b + lane] = x;
-    } else if (len <= kSortLds) {
-      int P = kWave;
-      while (P < len) P <<= 1;
-      for (int i = lane; i < P; i += kWave) buf[i] = i < len ? v[b + i] : 0x7fffffff;
-      wave_sync();
-      for (int k = 2; k <= P; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-          for (int i = lane; i < P; i += kWave) {
-            const int l = i ^ j;
-            if (l > i) {
-              const int32_t a = buf[i], c = buf[l];
-              const bool asc = (i & k) == 0;
-              if ((a > c) == asc) {
-                buf[i] = c;
-                buf[l] = a;
+    uint64_t longs = __ballot(mylen > kSortShort);
+    while (longs) {
+      const int src = __ffsll((unsigned long long)longs) - 1;
+      longs &= longs - 1;
+      const int64_t s = g0 + src;
+      const int64_t b = off[s];
+      const int64_t len = off[s + 1] - b;
+      if (len <= kWave) {
+        int32_t x = lane < len ? v[b + lane] : 0x7fffffff;
+        for (int k = 2; k <= kWave; k <<= 1)
+          for (int j = k >> 1; j > 0; j >>= 1) {
+            const int32_t y = __shfl_xor(x, j, kWave);
+            const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+            x = (lower == up) ? min(x, y) : max(x, y);
+          }
+        if (lane < len) v[b + lane] = x;
+      } else if (len <= kSortLds) {
+        int P = kWave;
+        while (P < len) P <<= 1;
+        for (int i = lane; i < P; i += kWave) buf[i] = i < len ? v[b + i] : 0x7fffffff;
+        wave_sync();
+        for (int k = 2; k <= P; k <<= 1)
+          for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < P; i += kWave) {
+              const int l = i ^ j;
+              if (l > i) {
+                const int32_t a = buf[i], c = buf[l];
+                const bool asc = (i & k) == 0;
+                if ((a > c) == asc) {
+                  buf[i] = c;
+                  buf[l] = a;
+                }
               }
             }
+            wave_sync();
           }
-          wave_sync();
-        }
-      for (int i = lane; i < len; i += kWave) v[b + i] = buf[i];
-      wave_sync();
-    } else if (lane == 0) {
-      heapsort_i32(v + b, len);
+        for (int i = lane; i < len; i += kWave) v[b + i] = buf[i];
+        wave_sync();
+      } else if (lane == 0) {
+        heapsort_i32(v + b, len);
+      }
     }
   }
 }
@@ -1387,6 +1545,8 @@ class Engine {
     cus_ = prop.multiProcessorCount;
     const char* kt = std::getenv("SSQ_KERNEL_TIMING");
     timing_ = kt && kt[0] == '1';
+    const char* tr = std::getenv("SSQ_TRACE_ROUNDS");  // per-round diagnostics on stderr
+    trace_ = tr && tr[0] == '1';
     // test hooks: smaller limits push short columns through the long-column paths
     if (const char* e = std::getenv("SSQ_TEST_SHORT_MAX")) short_max_ = std::max(2, std::min(kTile, std::atoi(e)));
     if (const char* e = std::getenv("SSQ_TEST_WINDOW_BITS"))
@@ -1427,7 +1587,7 @@ class Engine {
   void release() {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
                       &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
-                      &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_, &b_long_})
+                      &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
     if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
@@ -1444,7 +1604,9 @@ class Engine {
   int grid_for(int64_t n, int block = 256) const {
     int64_t g = (n + block - 1) / block;
     const int64_t cap = (int64_t)cus_ * 16;
-    return (int)std::max<int64_t>(1, std::min(g, cap));
+    g = std::min(g, cap);
+    if (g >= kXcds) g = (g + kXcds - 1) / kXcds * kXcds;  // XCD-contiguous element loops
+    return (int)std::max<int64_t>(1, g);
   }
   // Stable partition by a small key; totals go to device slot tot_[slot .. slot+nkeys).
   // With totals_host, they are also copied back (one sync); without, nothing syncs.
@@ -1482,9 +1644,10 @@ class Engine {
   std::string err_;
   DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
       b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
-      b_drem_, b_scan_, b_pmeta_, b_prem_, b_long_;
+      b_drem_, b_scan_, b_pmeta_, b_prem_;
   int64_t arena_hint_ = 0;
   bool timing_ = false;
+  bool trace_ = false;
   int32_t short_max_ = kTile;
   int32_t long_window_ = kLongBits;
   hipEvent_t ev_[8] = {};
@@ -1492,7 +1655,7 @@ class Engine {
 
 template <typename KeyF, typename EmitF>
 int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host, int slot) {
-  const int64_t nb = std::max<int64_t>(1, (n + kCompactBlock - 1) / kCompactBlock);
+  const int64_t nb = std::max<int64_t>(1, (n + kCompactItems - 1) / kCompactItems);
   uint32_t* counts;
   int rc = ensure(b_cnt_, (size_t)(nb * nkeys + 16), &counts);
   if (rc) return rc;
@@ -1503,7 +1666,7 @@ int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_
   } else {
     HIPCHK(hipMemsetAsync(counts, 0, sizeof(uint32_t) * nkeys, stream_));
   }
-  hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(1024), 0, stream_, nb, counts, nkeys, totals);
+  hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(kKeyScanThreads), 0, stream_, nb, counts, nkeys, totals);
   if (n > 0)
     hipLaunchKernelGGL((k_key_scatter<KeyF, EmitF>), dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key,
                        counts, nkeys, emit);
@@ -1518,9 +1681,10 @@ int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_
 struct KeyPendingDim {
   const uint8_t* state;
   const int32_t* dims;
+  int32_t nkeys;  // keys are dimensions 1..nkeys-1
   __device__ int operator()(int64_t c) const {
     const int32_t d = dims[c];
-    return (state[c] == ST_PENDING && d >= 1 && d < kMaxDims) ? d : -1;
+    return (state[c] == ST_PENDING && d >= 1 && d < nkeys) ? d : -1;
   }
 };
 struct EmitIndex {
@@ -1742,12 +1906,9 @@ int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int
   if (rc) return rc;
   HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
   hipLaunchKernelGGL(k_dual_fill, dim3(g), dim3(256), 0, stream_, n, colptr, rows, d.removed, *dcolptr, cnt, *drows);
-  int32_t* longs;
-  if ((rc = ensure(b_long_, (size_t)n + 16, &longs))) return rc;
-  uint32_t* nlong = (uint32_t*)(longs + n);
-  HIPCHK(hipMemsetAsync(nlong, 0, sizeof(uint32_t), stream_));
-  hipLaunchKernelGGL(k_sort_short, dim3(g), dim3(256), 0, stream_, n, *dcolptr, *drows, longs, nlong);
-  hipLaunchKernelGGL(k_sort_segments, dim3(cus_ * 8), dim3(256), 0, stream_, longs, nlong, *dcolptr, *drows);
+  const int64_t sort_groups = (n + 4 * kWave - 1) / (4 * kWave);
+  hipLaunchKernelGGL(k_sort_segments, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(sort_groups, (int64_t)cus_ * 32))),
+                     dim3(256), 0, stream_, n, *dcolptr, *drows);
   hipLaunchKernelGGL(k_dual_dims, dim3(g), dim3(256), 0, stream_, n, dims, maxdim, d.removed, *ddims, drem);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(d.removed, drem, (size_t)n, hipMemcpyDeviceToDevice, stream_));
@@ -1876,7 +2037,9 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     hipLaunchKernelGGL(k_compress, dim3(g), dim3(256), 0, stream_, d);
     HIPCHK(hipGetLastError());
     // pending lists per dimension and page bounds, read back in one round trip
-    rc = compact(n, kMaxDims, KeyPendingDim{d.state, d.dims}, EmitIndex{U}, nullptr, kTotPending);
+    const int pend_keys = std::min(kMaxDims, std::max(maxdim, 1) + 1);  // dims 1..maxdim hold pending columns
+    HIPCHK(hipMemsetAsync(tot_ + kTotPending, 0, sizeof(int64_t) * kMaxDims, stream_));
+    rc = compact(n, pend_keys, KeyPendingDim{d.state, d.dims, pend_keys}, EmitIndex{U}, nullptr, kTotPending);
     if (rc) return rc;
     int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
     hipLaunchKernelGGL(k_rowstat_init, dim3(1), dim3(64), 0, stream_, rowstat);
@@ -1958,7 +2121,17 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
                 HIPCHK(hipEventElapsedTime(&ms2, ev_[4 * t + 1], ev_[4 * t + 2]));
                 S->t_add_kernel_ms += ms;
                 S->t_add_long_kernel_ms += ms2;
+                if (trace_)
+                  std::fprintf(stderr, "[ssq] dim %d page %lld round %d: k_add %.3f ms, k_add_long %.3f ms\n", dim,
+                               (long long)S->n_pages, r - rounds_per_sync + t, ms, ms2);
               }
+            if (trace_)
+              std::fprintf(stderr,
+                           "[ssq] dim %d page %lld after round %d: window %lld, next worklist %u, last long list %u, "
+                           "additions so far %llu, longest chain %u, longest |a|+|b| %u\n",
+                           dim, (long long)S->n_pages, r - 1, (long long)wn, host_ctr_[cw::kW + (r & 1)],
+                           host_ctr_[cw::kLong], ((const unsigned long long*)(host_ctr_ + cw::kStats))[0], host_ctr_[cw::kChain],
+                           host_ctr_[cw::kChain + 1]);
             S->n_rounds += rounds_per_sync;
             if (host_ctr_[cw::kErr]) {
               oom = true;

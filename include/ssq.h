@@ -161,6 +161,15 @@ int ssq_vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim,
 /* The same matrix built on the handle's GPU (maxdim <= 2), copied back to the host. */
 int ssq_vr_build_gpu(ssq_handle* h, const float* pts, int64_t npts, int pdim, float r, int maxdim,
                      ssq_matrix* out);
+/* Persistence pairs of `count` point clouds' Rips complexes (same pdim, maxdim <= 2): the
+ * complexes are built on the GPU directly as one block-diagonal batch in HBM and reduced in
+ * one engine pass; nothing but points goes in and pairs come out.  outs[c] holds cloud c's
+ * pairs in the indices of its own matrix (the one ssq_vr_build(pts[c], npts[c], pdim, r[c],
+ * maxdim) returns).  No reference counterpart: the reference reduces one file per run
+ * (main.cu:322-350); this serves the point-cloud requests of a batched stream. */
+int ssq_batch_reduce_points(ssq_handle* h, int count, const float* const* pts, const int64_t* npts, int pdim,
+                            const float* r, int maxdim, const ssq_params* p, ssq_pairs* outs,
+                            ssq_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,12 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
     "ssq_vr_build_gpu": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
+    "ssq_batch_reduce_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),
+                                               ctypes.POINTER(ctypes.c_int64), ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                               ctypes.POINTER(ssq_params), ctypes.POINTER(ssq_pairs),
+                                               ctypes.POINTER(ssq_stats)]),
 }
 
 _lib = None

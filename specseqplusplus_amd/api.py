@@ -134,6 +134,26 @@ class Handle:
         return out
 
     # ---- device-resident buffers (HBM), owned by the caller through this handle ----
+    def batch_reduce_points(self, clouds, maxdim=2, p=None, stats=False):
+        """Pairs of many point clouds' Rips complexes, built and reduced as one batch on the
+        GPU.  clouds: list of (points (npts x pdim), r), all with the same pdim."""
+        p = p or params()
+        arrs = [np.ascontiguousarray(X, dtype=np.float32) for X, _ in clouds]
+        pdim = arrs[0].shape[1] if arrs else 1
+        if any(a.ndim != 2 or a.shape[1] != pdim for a in arrs):
+            raise ValueError("all clouds need the same dimension")
+        n = len(arrs)
+        fp = ctypes.POINTER(ctypes.c_float)
+        ptrs = (fp * n)(*[a.ctypes.data_as(fp) for a in arrs])
+        npts = (ctypes.c_int64 * n)(*[a.shape[0] for a in arrs])
+        rad = (ctypes.c_float * n)(*[float(r) for _, r in clouds])
+        outs = (L.ssq_pairs * n)()
+        st = L.ssq_stats()
+        L.check(L.lib().ssq_batch_reduce_points(self._h, n, ptrs, npts, pdim, rad, int(maxdim), ctypes.byref(p), outs,
+                                                ctypes.byref(st)), self.error())
+        res = [_pairs_from_c(outs[i]) for i in range(n)]
+        return (res, st.as_dict()) if stats else res
+
     def device_array(self, arr):
         """Copy a numpy array into a new device buffer; returns a DeviceArray."""
         arr = np.ascontiguousarray(arr)

@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -1250,6 +1251,32 @@ __device__ __forceinline__ int64_t find_block(const int64_t* off, int64_t count,
   return lo;
 }
 
+// Validation of host-supplied blocks before anything indexes with them: per column, the
+// offsets lie in its block's row array and its rows are strictly increasing and below the
+// block's cell count.  Offsets and rows are still block-local (unshifted).
+__global__ void k_check_blocks(int64_t n, int64_t count, const int64_t* base, const int64_t* nzb,
+                               const int64_t* colptr, const int32_t* rows, uint32_t* bad) {
+  GRID_STRIDE(g, n) {
+    const int64_t i = count > 1 ? find_block(base, count, g) : 0;
+    const int64_t c = g - base[i], n_i = base[i + 1] - base[i], nz_i = nzb[i + 1] - nzb[i];
+    const int64_t s = colptr[g], e = c + 1 < n_i ? colptr[g + 1] : nz_i;
+    bool ok = (c != 0 || s == 0) && s >= 0 && e >= s && e <= nz_i;
+    if (ok) {
+      const int32_t* R = rows + nzb[i];
+      int64_t prev = -1;
+      for (int64_t k = s; k < e; k++) {
+        const int64_t r = R[k];
+        if (r <= prev || r >= n_i) {
+          ok = false;
+          break;
+        }
+        prev = r;
+      }
+    }
+    if (!ok) *bad = 1u;
+  }
+}
+
 __global__ void k_batch_shift(int64_t n, int64_t nnz, int64_t count, const int64_t* base, const int64_t* nzb,
                               int64_t* colptr, int32_t* rows) {
   GRID_STRIDE(c, n) colptr[c] += nzb[find_block(base, count, c)];
@@ -1547,6 +1574,7 @@ class Engine {
     timing_ = kt && kt[0] == '1';
     const char* tr = std::getenv("SSQ_TRACE_ROUNDS");  // per-round diagnostics on stderr
     trace_ = tr && tr[0] == '1';
+    if (const char* e = std::getenv("SSQ_ROUNDS_PER_SYNC")) rounds_per_sync_ = std::max(1, std::min(8, std::atoi(e)));
     // test hooks: smaller limits push short columns through the long-column paths
     if (const char* e = std::getenv("SSQ_TEST_SHORT_MAX")) short_max_ = std::max(2, std::min(kTile, std::atoi(e)));
     if (const char* e = std::getenv("SSQ_TEST_WINDOW_BITS"))
@@ -1648,9 +1676,10 @@ class Engine {
   int64_t arena_hint_ = 0;
   bool timing_ = false;
   bool trace_ = false;
+  int rounds_per_sync_ = 2;  // add rounds launched between two reads of the worklist counters
   int32_t short_max_ = kTile;
   int32_t long_window_ = kLongBits;
-  hipEvent_t ev_[8] = {};
+  hipEvent_t ev_[32] = {};  // 4 per round, up to 8 rounds per sync
 };
 
 template <typename KeyF, typename EmitF>
@@ -2080,7 +2109,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     for (int k = 0; k < 5; k++) d.work[k] = W0 + k * wcap;
     const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * 12);
     const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
-    const int rounds_per_sync = 2;
+    const int rounds_per_sync = rounds_per_sync_;
     const int long_grid = cus_ * 2;
     bool oom = false;
     for (int32_t dim = maxdim; dim >= 1 && !oom; dim--) {
@@ -2213,104 +2242,239 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
 // ====================================================================================
 // C ABI
 // ====================================================================================
+struct PinBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
 struct Handle {
   Engine eng;
   std::string err;
-  DevBuf in_colptr, in_rows, in_dims, out_pairs, out_ess, offs;
+  DevBuf in_colptr, in_rows, in_dims, out_pairs, out_ess;
+  PinBuf pin_in, pin_out;
   VrBuilder* vr = nullptr;
   explicit Handle(int dev) : eng(dev) {}
   ~Handle() {
     if (vr) vr_builder_destroy(vr);
+    for (DevBuf* b : {&in_colptr, &in_rows, &in_dims, &out_pairs, &out_ess})
+      if (b->p) (void)hipFree(b->p);
+    for (PinBuf* b : {&pin_in, &pin_out})
+      if (b->p) (void)hipHostFree(b->p);
   }
 };
 
-static int upload(Handle* H, const int64_t* colptr, const int32_t* rows, const int32_t* dims, int64_t n,
-                  int64_t nnz, int64_t** dc, int32_t** dr, int32_t** dd, int64_t** dp, int64_t** de) {
-  auto grow = [&](DevBuf& b, size_t bytes) -> int {
-    bytes = std::max<size_t>(bytes, 8);
-    if (b.bytes >= bytes) return SSQ_OK;
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
-    if (hipMalloc(&b.p, bytes + bytes / 8) != hipSuccess) {
-      (void)hipGetLastError();
-      H->err = "device allocation failed";
-      return SSQ_ENOMEM;
-    }
-    b.bytes = bytes + bytes / 8;
-    return SSQ_OK;
+// Pinned host staging, grown on demand and kept by the handle.
+static int pin_grow(Handle* H, PinBuf& b, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 64);
+  if (b.bytes >= bytes) return SSQ_OK;
+  if (b.p) (void)hipHostFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  const size_t want = bytes + bytes / 4;
+  if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    H->err = "pinned host allocation failed";
+    return SSQ_ENOMEM;
+  }
+  b.bytes = want;
+  return SSQ_OK;
+}
+
+static int dev_grow(Handle* H, DevBuf& b, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 64);
+  if (b.bytes >= bytes) return SSQ_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  if (hipMalloc(&b.p, bytes + bytes / 8) != hipSuccess) {
+    (void)hipGetLastError();
+    H->err = "device allocation failed";
+    return SSQ_ENOMEM;
+  }
+  b.bytes = bytes + bytes / 8;
+  return SSQ_OK;
+}
+
+// memcpy of many pieces; large totals are split over host threads in 1 MiB chunks
+struct CopyJob {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+static void par_copy(const std::vector<CopyJob>& jobs) {
+  size_t total = 0;
+  for (const CopyJob& j : jobs) total += j.bytes;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int threads = total < (size_t(8) << 20) ? 1 : (int)std::min<unsigned>(8, hw);
+  if (threads == 1) {
+    for (const CopyJob& j : jobs)
+      if (j.bytes) std::memcpy(j.dst, j.src, j.bytes);
+    return;
+  }
+  constexpr size_t kChunk = size_t(1) << 20;
+  std::vector<CopyJob> pieces;
+  for (const CopyJob& j : jobs)
+    for (size_t o = 0; o < j.bytes; o += kChunk)
+      pieces.push_back({(char*)j.dst + o, (const char*)j.src + o, std::min(kChunk, j.bytes - o)});
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t k; (k = next.fetch_add(1)) < pieces.size();) std::memcpy(pieces[k].dst, pieces[k].src, pieces[k].bytes);
   };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; t++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+}
+
+static int split_pairs(Handle* H, const int64_t* dp, int64_t np, const int64_t* de, int64_t ne, const int64_t* base,
+                       int count, ssq_pairs* outs);
+
+// Reduce `count` host matrices as one block-diagonal batch (count == 1: a single matrix).
+// Host work is O(count) checks plus one staged copy; the per-nonzero validation runs on the
+// device before any kernel indexes with the input.  outs[i] receives block i's pairs in its
+// own indices.
+static int reduce_blocks(Handle* H, const ssq_matrix* const* mats, int count, const ssq_params& p, ssq_pairs* outs,
+                         ssq_stats* st) {
+  std::vector<int64_t> base((size_t)count + 1, 0), nzb((size_t)count + 1, 0);
+  for (int i = 0; i < count; i++) {
+    std::memset(&outs[i], 0, sizeof(ssq_pairs));
+    const ssq_matrix* m = mats[i];
+    if (!m || m->n < 0 || m->nnz < 0 || (m->n > 0 && (!m->col_ptr || !m->dims)) || (m->nnz > 0 && !m->rows)) {
+      H->err = "invalid matrix " + std::to_string(i);
+      return SSQ_EINVAL;
+    }
+    if (m->n > 0 && (m->col_ptr[0] != 0 || m->col_ptr[m->n] != m->nnz)) {
+      H->err = "malformed matrix " + std::to_string(i);
+      return SSQ_EFORMAT;
+    }
+    if (m->n == 0 && m->nnz != 0) {
+      H->err = "malformed matrix " + std::to_string(i);
+      return SSQ_EFORMAT;
+    }
+    base[i + 1] = base[i] + m->n;
+    nzb[i + 1] = nzb[i] + m->nnz;
+  }
+  const int64_t N = base[count], NNZ = nzb[count];
+  if (N >= 0x7fffffffLL) {
+    H->err = "more than 2^31 - 1 cells";
+    return SSQ_EINVAL;
+  }
+  // staging layout: colptr[N + 1] | base[count + 1] | nzb[count + 1] | rows[NNZ] | dims[N]
+  const size_t o_base = sizeof(int64_t) * (N + 1), o_nzb = o_base + sizeof(int64_t) * (count + 1),
+               o_rows = o_nzb + sizeof(int64_t) * (count + 1), o_dims = o_rows + sizeof(int32_t) * NNZ,
+               total = o_dims + sizeof(int32_t) * N;
   int rc;
-  if ((rc = grow(H->in_colptr, sizeof(int64_t) * (n + 1))) || (rc = grow(H->in_rows, sizeof(int32_t) * nnz)) ||
-      (rc = grow(H->in_dims, sizeof(int32_t) * n)) || (rc = grow(H->out_pairs, sizeof(int64_t) * 2 * n)) ||
-      (rc = grow(H->out_ess, sizeof(int64_t) * n)))
+  if ((rc = pin_grow(H, H->pin_in, total))) return rc;
+  char* stg = (char*)H->pin_in.p;
+  std::vector<CopyJob> jobs;
+  for (int i = 0; i < count; i++) {
+    const ssq_matrix* m = mats[i];
+    jobs.push_back({stg + sizeof(int64_t) * base[i], m->col_ptr, sizeof(int64_t) * m->n});
+    jobs.push_back({stg + o_rows + sizeof(int32_t) * nzb[i], m->rows, sizeof(int32_t) * m->nnz});
+    jobs.push_back({stg + o_dims + sizeof(int32_t) * base[i], m->dims, sizeof(int32_t) * m->n});
+  }
+  par_copy(jobs);
+  ((int64_t*)stg)[N] = NNZ;  // the shift below leaves it in place for the last block
+  std::memcpy(stg + o_base, base.data(), sizeof(int64_t) * (count + 1));
+  std::memcpy(stg + o_nzb, nzb.data(), sizeof(int64_t) * (count + 1));
+  if ((rc = dev_grow(H, H->in_colptr, o_rows)) || (rc = dev_grow(H, H->in_rows, sizeof(int32_t) * NNZ)) ||
+      (rc = dev_grow(H, H->in_dims, sizeof(int32_t) * N + 64)) ||
+      (rc = dev_grow(H, H->out_pairs, sizeof(int64_t) * 2 * N)) || (rc = dev_grow(H, H->out_ess, sizeof(int64_t) * N)))
     return rc;
   hipStream_t s = H->eng.stream();
-  if (hipMemcpyAsync(H->in_colptr.p, colptr, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
-      (nnz && hipMemcpyAsync(H->in_rows.p, rows, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s) != hipSuccess) ||
-      (n && hipMemcpyAsync(H->in_dims.p, dims, sizeof(int32_t) * n, hipMemcpyHostToDevice, s) != hipSuccess)) {
+  int64_t* dc = (int64_t*)H->in_colptr.p;  // colptr, then base and nzb
+  int64_t* dbase = dc + N + 1;
+  int64_t* dnzb = dbase + count + 1;
+  int32_t* dr = (int32_t*)H->in_rows.p;
+  int32_t* dd = (int32_t*)H->in_dims.p;
+  uint32_t* dbad = (uint32_t*)(dd + N + ((N & 1) ? 1 : 0));  // 8-byte aligned word after the dims
+  if (hipMemcpyAsync(dc, stg, o_rows, hipMemcpyHostToDevice, s) != hipSuccess ||
+      (NNZ && hipMemcpyAsync(dr, stg + o_rows, sizeof(int32_t) * NNZ, hipMemcpyHostToDevice, s) != hipSuccess) ||
+      (N && hipMemcpyAsync(dd, stg + o_dims, sizeof(int32_t) * N, hipMemcpyHostToDevice, s) != hipSuccess) ||
+      hipMemsetAsync(dbad, 0, sizeof(uint32_t), s) != hipSuccess) {
     H->err = "host to device copy failed";
     return SSQ_EDEVICE;
   }
-  *dc = (int64_t*)H->in_colptr.p;
-  *dr = (int32_t*)H->in_rows.p;
-  *dd = (int32_t*)H->in_dims.p;
-  *dp = (int64_t*)H->out_pairs.p;
-  *de = (int64_t*)H->out_ess.p;
-  return SSQ_OK;
-}
-
-static int check_matrix(const ssq_matrix* m) {
-  if (!m || m->n < 0 || m->nnz < 0 || (m->n > 0 && (!m->col_ptr || !m->dims))) return SSQ_EINVAL;
-  if (m->n == 0) return SSQ_OK;
-  if (m->col_ptr[0] != 0 || m->col_ptr[m->n] != m->nnz || (m->nnz > 0 && !m->rows)) return SSQ_EFORMAT;
-  for (int64_t c = 0; c < m->n; c++) {
-    if (m->col_ptr[c + 1] < m->col_ptr[c]) return SSQ_EFORMAT;
-    for (int64_t k = m->col_ptr[c]; k < m->col_ptr[c + 1]; k++) {
-      const int32_t r = m->rows[k];
-      if (r < 0 || r >= m->n || (k > m->col_ptr[c] && m->rows[k - 1] >= r)) return SSQ_EFORMAT;
+  if (N > 0) {
+    const int grid = (int)std::max<int64_t>(8, std::min<int64_t>((N + 255) / 256, 4096) / 8 * 8);
+    hipLaunchKernelGGL(k_check_blocks, dim3(grid), dim3(256), 0, s, N, (int64_t)count, dbase, dnzb, dc, dr, dbad);
+    uint32_t bad = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&bad, dbad, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      H->err = "device validation failed";
+      return SSQ_EDEVICE;
+    }
+    if (bad) {
+      H->err = count > 1 ? "malformed matrix in batch (row out of range or rows not strictly increasing)"
+                         : "malformed matrix (row out of range or rows not strictly increasing)";
+      return SSQ_EFORMAT;
+    }
+    if (count > 1) {
+      const int64_t work = std::max<int64_t>(N, NNZ);
+      const int grid2 = (int)std::max<int64_t>(8, std::min<int64_t>((work + 255) / 256, 4096) / 8 * 8);
+      hipLaunchKernelGGL(k_batch_shift, dim3(grid2), dim3(256), 0, s, N, NNZ, (int64_t)count, dbase, dnzb, dc, dr);
+      if (hipGetLastError() != hipSuccess) return SSQ_EDEVICE;
     }
   }
-  return SSQ_OK;
-}
-
-// reduce host CSC arrays; pairs returned in the caller's indices
-static int reduce_host(Handle* H, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows,
-                       const int32_t* dims, const ssq_params& p, std::vector<int64_t>& pairs,
-                       std::vector<int64_t>& ess, ssq_stats* st) {
-  int64_t *dc, *dp, *de;
-  int32_t *dr, *dd;
-  int rc = upload(H, colptr, rows, dims, n, nnz, &dc, &dr, &dd, &dp, &de);
-  if (rc) return rc;
   int64_t np = 0, ne = 0;
-  rc = H->eng.reduce_device(n, nnz, dc, dr, dd, p, dp, &np, st, de, &ne);
+  int64_t* dp = (int64_t*)H->out_pairs.p;
+  int64_t* de = (int64_t*)H->out_ess.p;
+  rc = H->eng.reduce_device(N, NNZ, dc, dr, dd, p, dp, &np, st, de, &ne);
   if (rc) {
     H->err = H->eng.error();
     return rc;
   }
-  pairs.resize((size_t)(2 * np));
-  ess.resize((size_t)ne);
+  return split_pairs(H, dp, np, de, ne, base.data(), count, outs);
+}
+
+// Copy the batch's pairs and essentials to the host and give each block its own run, in its
+// own indices (pairs are sorted by birth and essentials ascending).
+static int split_pairs(Handle* H, const int64_t* dp, int64_t np, const int64_t* de, int64_t ne, const int64_t* base,
+                       int count, ssq_pairs* outs) {
   hipStream_t s = H->eng.stream();
-  if ((np && hipMemcpyAsync(pairs.data(), dp, sizeof(int64_t) * 2 * np, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-      (ne && hipMemcpyAsync(ess.data(), de, sizeof(int64_t) * ne, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+  int rc;
+  if ((rc = pin_grow(H, H->pin_out, sizeof(int64_t) * (2 * np + ne)))) return rc;
+  int64_t* hp = (int64_t*)H->pin_out.p;
+  int64_t* he = hp + 2 * np;
+  if ((np && hipMemcpyAsync(hp, dp, sizeof(int64_t) * 2 * np, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (ne && hipMemcpyAsync(he, de, sizeof(int64_t) * ne, hipMemcpyDeviceToHost, s) != hipSuccess) ||
       hipStreamSynchronize(s) != hipSuccess) {
     H->err = "device to host copy failed";
     return SSQ_EDEVICE;
   }
-  return SSQ_OK;
-}
-
-static int fill_pairs(const std::vector<int64_t>& pairs, const std::vector<int64_t>& ess, ssq_pairs* out) {
-  out->np = (int64_t)pairs.size() / 2;
-  out->ne = (int64_t)ess.size();
-  out->pairs = (int64_t*)std::malloc(sizeof(int64_t) * std::max<size_t>(pairs.size(), 1));
-  out->essential = (int64_t*)std::malloc(sizeof(int64_t) * std::max<size_t>(ess.size(), 1));
-  if (!out->pairs || !out->essential) {
-    ssq_pairs_free(out);
-    return SSQ_ENOMEM;
+  int64_t pi = 0, ei = 0;
+  for (int i = 0; i < count; i++) {
+    int64_t lo = pi, hi = np;  // first pair with birth >= base[i + 1]
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (hp[2 * mid] < base[i + 1]) lo = mid + 1;
+      else hi = mid;
+    }
+    const int64_t pj = lo;
+    int64_t elo = ei, ehi = ne;
+    while (elo < ehi) {
+      const int64_t mid = (elo + ehi) >> 1;
+      if (he[mid] < base[i + 1]) elo = mid + 1;
+      else ehi = mid;
+    }
+    const int64_t ej = elo;
+    ssq_pairs* o = &outs[i];
+    o->np = pj - pi;
+    o->ne = ej - ei;
+    o->pairs = (int64_t*)std::malloc(sizeof(int64_t) * std::max<int64_t>(2 * o->np, 1));
+    o->essential = (int64_t*)std::malloc(sizeof(int64_t) * std::max<int64_t>(o->ne, 1));
+    if (!o->pairs || !o->essential) {
+      for (int k = 0; k <= i; k++) ssq_pairs_free(&outs[k]);
+      H->err = "host allocation failed";
+      return SSQ_ENOMEM;
+    }
+    const int64_t b = base[i];
+    for (int64_t k = 0; k < 2 * o->np; k++) o->pairs[k] = hp[2 * pi + k] - b;
+    for (int64_t k = 0; k < o->ne; k++) o->essential[k] = he[ei + k] - b;
+    pi = pj;
+    ei = ej;
   }
-  if (!pairs.empty()) std::memcpy(out->pairs, pairs.data(), sizeof(int64_t) * pairs.size());
-  if (!ess.empty()) std::memcpy(out->essential, ess.data(), sizeof(int64_t) * ess.size());
   return SSQ_OK;
 }
 
@@ -2343,8 +2507,7 @@ extern "C" int ssq_create(int device, ssq_handle** out) {
 
 extern "C" int ssq_destroy(ssq_handle* h) {
   if (!h) return SSQ_EINVAL;
-  for (DevBuf* b : {&h->h->in_colptr, &h->h->in_rows, &h->h->in_dims, &h->h->out_pairs, &h->h->out_ess, &h->h->offs})
-    if (b->p) (void)hipFree(b->p);
+  if (hipSetDevice(h->h->eng.device()) != hipSuccess) (void)hipGetLastError();
   delete h->h;
   delete h;
   return SSQ_OK;
@@ -2357,15 +2520,10 @@ extern "C" int ssq_reduce(ssq_handle* h, const ssq_matrix* m, const ssq_params* 
   if (!h || !p || !out) return SSQ_EINVAL;
   std::memset(out, 0, sizeof(*out));
   h->h->err.clear();
-  int rc = check_matrix(m);
-  if (rc) {
-    h->h->err = "malformed matrix";
-    return rc;
-  }
-  std::vector<int64_t> pairs, ess;
-  rc = reduce_host(h->h, m->n, m->nnz, m->col_ptr, m->rows, m->dims, *p, pairs, ess, stats);
-  if (rc) return rc;
-  return fill_pairs(pairs, ess, out);
+  if (!m) return SSQ_EINVAL;
+  if (hipSetDevice(h->h->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  const ssq_matrix* one[1] = {m};
+  return reduce_blocks(h->h, one, 1, *p, out, stats);
 }
 
 extern "C" int ssq_reduce_device(ssq_handle* h, int64_t n, int64_t nnz, const int64_t* d_col_ptr,
@@ -2382,91 +2540,8 @@ extern "C" int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, in
                                 ssq_pairs* outs, ssq_stats* stats) {
   if (!h || !p || !outs || count < 0 || (count > 0 && !mats)) return SSQ_EINVAL;
   h->h->err.clear();
-  // block-diagonal layout: complex i's cells are shifted by base[i]
-  std::vector<int64_t> base((size_t)count + 1, 0), nzb((size_t)count + 1, 0);
-  for (int i = 0; i < count; i++) {
-    std::memset(&outs[i], 0, sizeof(ssq_pairs));
-    int rc = check_matrix(mats[i]);
-    if (rc) {
-      h->h->err = "malformed matrix " + std::to_string(i);
-      return rc;
-    }
-    base[i + 1] = base[i] + mats[i]->n;
-    nzb[i + 1] = nzb[i] + mats[i]->nnz;
-  }
-  const int64_t N = base[count], NNZ = nzb[count];
-  if (N >= 0x7fffffffLL) {
-    h->h->err = "batch exceeds 2^31 cells";
-    return SSQ_EINVAL;
-  }
-  // unshifted concatenation on the host (memcpy only); offsets are applied on the device
-  std::vector<int64_t> cp((size_t)N + 1);
-  std::vector<int32_t> rw((size_t)std::max<int64_t>(NNZ, 1)), dm((size_t)std::max<int64_t>(N, 1));
-  for (int i = 0; i < count; i++) {
-    const ssq_matrix* m = mats[i];
-    if (m->n) std::memcpy(&cp[(size_t)base[i]], m->col_ptr, sizeof(int64_t) * m->n);
-    if (m->nnz) std::memcpy(&rw[(size_t)nzb[i]], m->rows, sizeof(int32_t) * m->nnz);
-    if (m->n) std::memcpy(&dm[(size_t)base[i]], m->dims, sizeof(int32_t) * m->n);
-  }
-  cp[(size_t)N] = 0;
-  std::vector<int64_t> pairs, ess;
-  int64_t *dc, *dp, *de;
-  int32_t *dr, *dd;
-  int rc = upload(h->h, cp.data(), rw.data(), dm.data(), N, NNZ, &dc, &dr, &dd, &dp, &de);
-  if (rc) return rc;
-  {
-    DevBuf& ob = h->h->offs;
-    const size_t bytes = sizeof(int64_t) * 2 * ((size_t)count + 1);
-    if (ob.bytes < bytes) {
-      if (ob.p) (void)hipFree(ob.p);
-      ob.p = nullptr;
-      ob.bytes = 0;
-      if (hipMalloc(&ob.p, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        return SSQ_ENOMEM;
-      }
-      ob.bytes = bytes;
-    }
-    int64_t* dbase = (int64_t*)ob.p;
-    int64_t* dnzb = dbase + count + 1;
-    hipStream_t st = h->h->eng.stream();
-    if (hipMemcpyAsync(dbase, base.data(), sizeof(int64_t) * (count + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dnzb, nzb.data(), sizeof(int64_t) * (count + 1), hipMemcpyHostToDevice, st) != hipSuccess)
-      return SSQ_EDEVICE;
-    const int64_t work = std::max<int64_t>(N, NNZ);
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, 4096));
-    hipLaunchKernelGGL(k_batch_shift, dim3(grid), dim3(256), 0, st, N, NNZ, (int64_t)count, dbase, dnzb, dc, dr);
-    if (hipGetLastError() != hipSuccess) return SSQ_EDEVICE;
-  }
-  int64_t np = 0, ne = 0;
-  rc = h->h->eng.reduce_device(N, NNZ, dc, dr, dd, *p, dp, &np, stats, de, &ne);
-  if (rc) {
-    h->h->err = h->h->eng.error();
-    return rc;
-  }
-  pairs.resize((size_t)(2 * np));
-  ess.resize((size_t)ne);
-  {
-    hipStream_t st = h->h->eng.stream();
-    if ((np && hipMemcpyAsync(pairs.data(), dp, sizeof(int64_t) * 2 * np, hipMemcpyDeviceToHost, st) != hipSuccess) ||
-        (ne && hipMemcpyAsync(ess.data(), de, sizeof(int64_t) * ne, hipMemcpyDeviceToHost, st) != hipSuccess) ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return SSQ_EDEVICE;
-  }
-  // pairs are sorted by birth, so each complex's pairs are a contiguous run
-  size_t pi = 0, ei = 0;
-  for (int i = 0; i < count; i++) {
-    std::vector<int64_t> pp, ee;
-    while (pi < pairs.size() && pairs[pi] < base[i + 1]) {
-      pp.push_back(pairs[pi] - base[i]);
-      pp.push_back(pairs[pi + 1] - base[i]);
-      pi += 2;
-    }
-    while (ei < ess.size() && ess[ei] < base[i + 1]) ee.push_back(ess[ei++] - base[i]);
-    rc = fill_pairs(pp, ee, &outs[i]);
-    if (rc) return rc;
-  }
-  return SSQ_OK;
+  if (hipSetDevice(h->h->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  return reduce_blocks(h->h, mats, count, *p, outs, stats);
 }
 
 extern "C" int ssq_device_alloc(ssq_handle* h, size_t bytes, void** dptr) {
@@ -2511,6 +2586,39 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
   if (!h) return SSQ_EINVAL;
   if (hipSetDevice(h->h->eng.device()) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return SSQ_EDEVICE;
   return SSQ_OK;
+}
+
+extern "C" int ssq_batch_reduce_points(ssq_handle* h, int count, const float* const* pts, const int64_t* npts,
+                                       int pdim, const float* r, int maxdim, const ssq_params* p, ssq_pairs* outs,
+                                       ssq_stats* stats) {
+  if (!h || !p || !outs || count < 1 || !pts || !npts || !r || pdim < 1 || maxdim < 0 || maxdim > 2)
+    return SSQ_EINVAL;
+  for (int i = 0; i < count; i++) std::memset(&outs[i], 0, sizeof(ssq_pairs));
+  Handle* H = h->h;
+  H->err.clear();
+  if (!H->vr && !(H->vr = vr_builder_create())) return SSQ_ENOMEM;
+  if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  std::vector<int64_t> base((size_t)count + 1);
+  const int64_t* dc;
+  const int32_t *dr, *dd;
+  int64_t n = 0, nnz = 0;
+  int rc = vr_builder_build_batch(H->vr, H->eng.stream(), count, pts, npts, pdim, r, maxdim, &dc, &dr, &dd, &n, &nnz,
+                                  base.data());
+  if (rc) {
+    H->err = rc == SSQ_EINVAL ? "invalid point clouds" : "GPU Rips build failed";
+    return rc;
+  }
+  if ((rc = dev_grow(H, H->out_pairs, sizeof(int64_t) * 2 * n)) || (rc = dev_grow(H, H->out_ess, sizeof(int64_t) * n)))
+    return rc;
+  int64_t np = 0, ne = 0;
+  int64_t* dp = (int64_t*)H->out_pairs.p;
+  int64_t* de = (int64_t*)H->out_ess.p;
+  rc = H->eng.reduce_device(n, nnz, dc, dr, dd, *p, dp, &np, stats, de, &ne);
+  if (rc) {
+    H->err = H->eng.error();
+    return rc;
+  }
+  return split_pairs(H, dp, np, de, ne, base.data(), count, outs);
 }
 
 extern "C" int ssq_vr_build_gpu(ssq_handle* h, const float* pts, int64_t npts, int pdim, float r, int maxdim,

@@ -28,5 +28,11 @@ void vr_builder_destroy(VrBuilder* b);
 int vr_builder_build(VrBuilder* b, hipStream_t s, const float* h_pts, int64_t npts, int pdim, float r, int maxdim,
                      const int64_t** d_colptr, const int32_t** d_rows, const int32_t** d_dims, int64_t* n_out,
                      int64_t* nnz_out);
+// Many clouds (same pdim) in one build: the output is their block-diagonal matrix, cloud c's
+// cells in rows/columns [cell_base[c], cell_base[c + 1]) in its own (filtration, dim, lex)
+// order.  cell_base (count + 1 entries) may be null.
+int vr_builder_build_batch(VrBuilder* b, hipStream_t s, int count, const float* const* h_pts, const int64_t* npts,
+                           int pdim, const float* radius, int maxdim, const int64_t** d_colptr, const int32_t** d_rows,
+                           const int32_t** d_dims, int64_t* n_out, int64_t* nnz_out, int64_t* cell_base);
 
 }  // namespace ssq

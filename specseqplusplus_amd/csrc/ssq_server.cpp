@@ -96,6 +96,8 @@ struct ssq_server {
         int64_t cells = 0;
         while (!Q.empty() && (int)batch.size() < cfg.max_batch) {
           Request* q = Q.front();
+          // point clouds of one batch share their dimension and simplex dimension
+          if (kind == 1 && !batch.empty() && (q->pdim != batch[0]->pdim || q->maxdim != batch[0]->maxdim)) break;
           if (in_flight_bytes + q->est_bytes > cfg.mem_budget_bytes && (in_flight_bytes > 0 || !batch.empty()))
             break;
           if (cfg.max_batch_cells > 0 && !batch.empty() && cells + q->est_cells > cfg.max_batch_cells) break;
@@ -112,12 +114,12 @@ struct ssq_server {
       }
       std::vector<ssq_matrix> mats(batch.size());
       std::vector<int> st(batch.size(), SSQ_OK);
-      for (size_t i = 0; i < batch.size(); i++) {
+      const bool gpu_points = batch_kind == 1 && batch[0]->maxdim <= 2;
+      for (size_t i = 0; i < batch.size() && !gpu_points; i++) {
         Request* q = batch[i];
         if (q->is_points) {
-          // Rips matrices of point requests are built on this worker's GPU stream
-          st[i] = (h && q->maxdim <= 2) ? ssq_vr_build_gpu(h, q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &mats[i])
-                                        : ssq::vr_build(q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &mats[i]);
+          // dimension 3: the Rips matrix is built on the host
+          st[i] = ssq::vr_build(q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &mats[i]);
         } else {
           mats[i].n = (int64_t)q->dims.size();
           mats[i].nnz = (int64_t)q->rows.size();
@@ -135,7 +137,21 @@ struct ssq_server {
         }
       std::vector<ssq_pairs> outs(ptrs.size());
       int brc = SSQ_OK;
-      if (!ptrs.empty()) {
+      if (gpu_points) {
+        // the Rips matrices are built on this worker's stream straight into one batch in HBM
+        std::vector<const float*> pp(batch.size());
+        std::vector<int64_t> pn(batch.size());
+        std::vector<float> pr(batch.size());
+        for (size_t i = 0; i < batch.size(); i++) {
+          pp[i] = batch[i]->pts.data();
+          pn[i] = batch[i]->npts;
+          pr[i] = batch[i]->r;
+        }
+        outs.resize(batch.size());
+        brc = h ? ssq_batch_reduce_points(h, (int)batch.size(), pp.data(), pn.data(), batch[0]->pdim, pr.data(),
+                                          batch[0]->maxdim, &cfg.points_params, outs.data(), nullptr)
+                : SSQ_EDEVICE;
+      } else if (!ptrs.empty()) {
         const ssq_params* bp = batch_kind ? &cfg.points_params : &cfg.params;
         brc = h ? ssq_batch_reduce(h, ptrs.data(), (int)ptrs.size(), bp, outs.data(), nullptr) : SSQ_EDEVICE;
       }
@@ -162,7 +178,7 @@ struct ssq_server {
           outstanding--;
         }
       }
-      for (size_t i = 0; i < batch.size(); i++)
+      for (size_t i = 0; i < batch.size() && !gpu_points; i++)
         if (batch[i]->is_points && st[i] == SSQ_OK) ssq_matrix_free(&mats[i]);
       for (Request* q : batch) delete q;
       cv_done.notify_all();

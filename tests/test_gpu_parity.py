@@ -11,6 +11,7 @@ import fixtures as F
 from oracle import oracle as O
 
 import specseqplusplus_amd as S
+from specseqplusplus_amd import _lib
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -171,6 +172,44 @@ def test_invalid_parameters_are_rejected(h):
             h.reduce(as_product(mo), S.params(**kw))
 
 
+def _malformed(kind):
+    """A small valid complex with one planted defect (rows and offsets checked on the device)."""
+    mo = c1_matrix()
+    cp, rows, dims = mo.colptr.copy(), mo.rows.copy(), mo.dims.copy()
+    c = int(np.flatnonzero(np.diff(cp) >= 2)[5])  # a column with at least two rows
+    if kind == "row_out_of_range":
+        rows[cp[c + 1] - 1] = len(dims)
+    elif kind == "negative_row":
+        rows[cp[c]] = -1
+    elif kind == "unsorted":
+        rows[cp[c]], rows[cp[c] + 1] = rows[cp[c] + 1], rows[cp[c]]
+    elif kind == "duplicate":
+        rows[cp[c] + 1] = rows[cp[c]]
+    elif kind == "offsets_not_monotone":
+        cp[c + 1] = cp[c] - 1
+    elif kind == "last_offset":
+        cp[-1] -= 1
+    return S.Matrix(cp, rows, dims)
+
+
+@pytest.mark.parametrize("kind", ["row_out_of_range", "negative_row", "unsorted", "duplicate",
+                                  "offsets_not_monotone", "last_offset"])
+def test_malformed_matrix_is_rejected(h, kind):
+    bad = _malformed(kind)
+    good = as_product(c1_matrix())
+    ref = h.reduce(good)[0]
+    with pytest.raises(S.SSQError) as e:
+        h.reduce(bad)
+    assert e.value.status == _lib.SSQ_EFORMAT
+    with pytest.raises(S.SSQError) as e:  # the defect is found inside a batch too
+        h.batch_reduce([good, bad, good])
+    assert e.value.status == _lib.SSQ_EFORMAT
+    # the handle stays usable and exact
+    assert np.array_equal(h.reduce(good)[0], ref)
+    outs = h.batch_reduce([good, good])
+    assert all(np.array_equal(o[0], ref) for o in outs)
+
+
 def _circle(n, seed):
     rng = np.random.default_rng(seed)
     t = rng.random(n) * 2 * np.pi
@@ -219,3 +258,26 @@ def test_gpu_rips_builder_matches_host(h, case):
         b = h.vr_build_gpu(X, r, maxdim)
         assert np.array_equal(a.dims, b.dims) and np.array_equal(a.colptr, b.colptr), (case, maxdim)
         assert np.array_equal(a.rows, b.rows), (case, maxdim)
+
+
+@pytest.mark.parametrize("maxdim", [1, 2])
+def test_batch_reduce_points_equals_alone(h, maxdim):
+    """Clouds built on the GPU as one block-diagonal batch and reduced together give, per
+    cloud, the pairs of its own host-built Rips matrix reduced alone (and the oracle's)."""
+    from scipy.spatial.distance import pdist
+    from specseqplusplus_amd import configs as CF
+    rng = np.random.default_rng(40 + maxdim)
+    clouds = []
+    for i in range(13):
+        n = [0, 1, 2, 37, 90, 150, 64, 200, 5, 120, 80, 33, 170][i]
+        X = CF.cloud(rng, i % 3, n).astype(np.float32) if n else np.zeros((0, 3), np.float32)
+        r = float(np.percentile(pdist(X.astype(np.float64)), 25)) if n > 1 else 0.5
+        clouds.append((X, r))
+    for p in [S.params(dualize=1, ss_block_size=1 << 30), S.params()]:
+        outs = h.batch_reduce_points(clouds, maxdim, p)
+        for (X, r), (pairs, ess) in zip(clouds, outs):
+            m = S.vr_build(X, r, maxdim)
+            ref, ref_ess = h.reduce(m, p)
+            assert np.array_equal(pairs, ref) and np.array_equal(ess, ref_ess), (len(X), maxdim)
+            mo = O.Matrix(m.colptr, m.rows, m.dims)
+            assert np.array_equal(pairs, O.twist_pairs(mo))

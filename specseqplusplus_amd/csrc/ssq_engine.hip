@@ -1601,7 +1601,7 @@ class Engine {
       if (b.p) (void)hipFree(b.p);
       b.p = nullptr;
       b.bytes = 0;
-      const size_t alloc = need + need / 8;
+      const size_t alloc = need + need / 2;  // geometric growth: a free synchronizes the device
       if (hipMalloc(&b.p, alloc) != hipSuccess) {
         (void)hipGetLastError();
         err_ = "device allocation of " + std::to_string(alloc) + " bytes failed";
@@ -2270,7 +2270,7 @@ static int pin_grow(Handle* H, PinBuf& b, size_t bytes) {
   if (b.p) (void)hipHostFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
-  const size_t want = bytes + bytes / 4;
+  const size_t want = bytes + bytes / 2;
   if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     H->err = "pinned host allocation failed";
@@ -2286,12 +2286,12 @@ static int dev_grow(Handle* H, DevBuf& b, size_t bytes) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
-  if (hipMalloc(&b.p, bytes + bytes / 8) != hipSuccess) {
+  if (hipMalloc(&b.p, bytes + bytes / 2) != hipSuccess) {
     (void)hipGetLastError();
     H->err = "device allocation failed";
     return SSQ_ENOMEM;
   }
-  b.bytes = bytes + bytes / 8;
+  b.bytes = bytes + bytes / 2;
   return SSQ_OK;
 }
 

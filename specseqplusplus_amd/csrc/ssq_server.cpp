@@ -10,6 +10,7 @@
 // mixes rows of different requests.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -56,6 +57,78 @@ int64_t estimate_points(int64_t npts, int maxdim) {
   return estimate_bytes(npts + e + t, 2 * e + 3 * t);
 }
 
+// Triangulated k x k grid (vertices, then edges, then triangles: a valid filtration) with
+// about `cells` cells.  Reducing it sizes a worker's device buffers before requests arrive,
+// so that no buffer growth (whose free synchronizes the device) lands in a served batch.
+void grid_complex(int64_t cells, std::vector<int64_t>& cp, std::vector<int32_t>& rows, std::vector<int32_t>& dims) {
+  int64_t k = 2;
+  while (6 * (k + 1) * (k + 1) <= cells) k++;
+  const int64_t V = k * k;
+  auto vid = [k](int64_t x, int64_t y) { return (int32_t)(y * k + x); };
+  std::vector<std::pair<int32_t, int32_t>> E;
+  for (int64_t y = 0; y < k; y++)
+    for (int64_t x = 0; x < k; x++) {
+      if (x + 1 < k) E.push_back({vid(x, y), vid(x + 1, y)});
+      if (y + 1 < k) E.push_back({vid(x, y), vid(x, y + 1)});
+      if (x + 1 < k && y + 1 < k) E.push_back({vid(x, y), vid(x + 1, y + 1)});
+    }
+  std::sort(E.begin(), E.end());
+  auto eid = [&](int32_t a, int32_t b) {
+    return (int32_t)(V + (std::lower_bound(E.begin(), E.end(), std::make_pair(a, b)) - E.begin()));
+  };
+  rows.clear();
+  dims.assign((size_t)V, 0);
+  for (auto& e : E) {
+    rows.push_back(e.first);
+    rows.push_back(e.second);
+    dims.push_back(1);
+  }
+  for (int64_t y = 0; y + 1 < k; y++)
+    for (int64_t x = 0; x + 1 < k; x++) {
+      const int32_t a = vid(x, y), b = vid(x + 1, y), c = vid(x, y + 1), d = vid(x + 1, y + 1);
+      int32_t t1[3] = {eid(a, b), eid(a, d), eid(b, d)}, t2[3] = {eid(a, c), eid(a, d), eid(c, d)};
+      for (int32_t* t : {t1, t2}) {
+        std::sort(t, t + 3);
+        rows.insert(rows.end(), t, t + 3);
+        dims.push_back(2);
+      }
+    }
+  cp.assign(dims.size() + 1, 0);
+  for (size_t c = 0; c < dims.size(); c++) cp[c + 1] = cp[c] + (dims[c] == 0 ? 0 : dims[c] + 1);
+}
+
+void warm_up(ssq_handle* h, const ssq_server_config& cfg) {
+  const int64_t cells = std::min<int64_t>(cfg.max_batch_cells > 0 ? cfg.max_batch_cells : 1000000, 4000000);
+  std::vector<int64_t> cp;
+  std::vector<int32_t> rows, dims;
+  grid_complex(cells, cp, rows, dims);
+  ssq_matrix m{(int64_t)dims.size(), (int64_t)rows.size(), cp.data(), rows.data(), dims.data()};
+  const ssq_matrix* one[1] = {&m};
+  for (const ssq_params* p : {&cfg.params, &cfg.points_params}) {
+    ssq_pairs out;
+    if (ssq_batch_reduce(h, one, 1, p, &out, nullptr) == SSQ_OK) ssq_pairs_free(&out);
+  }
+  // point clouds: random points in the unit cube, about 20 neighbours each
+  constexpr int kClouds = 16, kPts = 2048;
+  std::vector<float> pts((size_t)kClouds * kPts * 3);
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+  for (float& v : pts) {  // splitmix64
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    v = (float)((z ^ (z >> 31)) >> 40) / (float)(1ull << 24);
+  }
+  std::vector<const float*> pp(kClouds);
+  std::vector<int64_t> pn(kClouds, kPts);
+  std::vector<float> pr(kClouds, 0.135f);
+  for (int c = 0; c < kClouds; c++) pp[c] = pts.data() + (size_t)c * kPts * 3;
+  std::vector<ssq_pairs> outs(kClouds);
+  if (ssq_batch_reduce_points(h, kClouds, pp.data(), pn.data(), 3, pr.data(), 2, &cfg.points_params, outs.data(),
+                              nullptr) == SSQ_OK)
+    for (auto& o : outs) ssq_pairs_free(&o);
+}
+
 }  // namespace
 
 struct ssq_server {
@@ -68,6 +141,7 @@ struct ssq_server {
   int64_t next_id = 1;
   int64_t in_flight_bytes = 0;
   int64_t outstanding = 0;
+  int ready = 0;  // workers that finished their warm-up
   bool stop = false;
   int init_status = SSQ_OK;
 
@@ -77,7 +151,14 @@ struct ssq_server {
     if (rc) {
       std::lock_guard<std::mutex> g(mu);
       init_status = rc;
+    } else {
+      warm_up(h, cfg);
     }
+    {
+      std::lock_guard<std::mutex> g(mu);
+      ready++;
+    }
+    cv_done.notify_all();
     for (;;) {
       std::vector<Request*> batch;
       int batch_kind = 0;
@@ -221,6 +302,10 @@ extern "C" int ssq_server_create(const ssq_server_config* cfg, ssq_server** out)
     s->cfg.mem_budget_bytes = (int64_t)(fr / 2);
   }
   for (int w = 0; w < cfg->streams; w++) s->workers.emplace_back(&ssq_server::worker_main, s, w);
+  {  // requests are accepted once every worker has sized its buffers
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->ready == cfg->streams; });
+  }
   *out = s;
   return SSQ_OK;
 }

@@ -228,11 +228,11 @@ bool grow(void*& p, size_t& bytes, size_t count) {
   if (p) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
-  if (hipMalloc(&p, need + need / 8) != hipSuccess) {
+  if (hipMalloc(&p, need + need / 2) != hipSuccess) {  // geometric growth: a free synchronizes the device
     (void)hipGetLastError();
     return false;
   }
-  bytes = need + need / 8;
+  bytes = need + need / 2;
   return true;
 }
 

@@ -352,7 +352,8 @@ def main():
         idx = np.random.default_rng(2).permutation(args.complexes)[:args.plain]
         dev_mats = [(h.upload(mats[i]), plans[class_of[i]]["p"]) for i in idx]
         d_pairs = h.device_empty(2 * max(mats[i].n for i in idx), np.int64)
-        h.reduce_resident(dev_mats[0][0], d_pairs, dev_mats[0][1])
+        big = int(np.argmax([mats[i].n for i in idx]))  # size the engine's buffers first
+        h.reduce_resident(dev_mats[big][0], d_pairs, dev_mats[big][1])
         h.synchronize()
         tt = time.perf_counter()
         for dm, pp in dev_mats:
@@ -376,7 +377,10 @@ def main():
         kinds = rng.integers(0, 2, n_plain)
         picks = rng.integers(0, 1 << 30, n_plain)
         pp_pts, pp_mat = S.params(**SERVED_PARAMS), S.params(**SERVED_MATRIX_PARAMS)
-        h.reduce(pool_mats[0], pp_mat)
+        # size the engine's buffers with the largest request of each kind first
+        h.reduce(max(pool_mats, key=lambda m: m.n), pp_mat)
+        Xb, rb = max(pool_pts, key=lambda c: len(c[0]))
+        h.reduce(h.vr_build_gpu(Xb, rb, 2), pp_pts)
         tt = time.perf_counter()
         for k in range(n_plain):
             if kinds[k] == 0:

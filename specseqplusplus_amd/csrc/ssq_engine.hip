@@ -107,6 +107,7 @@ constexpr int kLvl = 52;       // [52..55] level slot counters
 constexpr int kErr = 56;       // store exhausted
 constexpr int kMaxDim = 58;    // [58] max dimension, [59] bad dimension flag
 constexpr int kChain = 60;     // [60] longest chain, [61] largest |a|+|b|
+constexpr int kNegDims = 62;   // bit d set: some negative cell has dimension d
 constexpr int kWords = 64;
 }  // namespace cw
 
@@ -262,12 +263,34 @@ __global__ void k_clearing(Dev d) {
   }
 }
 
+// Dimensions that hold a negative cell (one bit each), for k_compress.
+__global__ void k_negdims(Dev d) {
+  uint32_t bits = 0;
+  GRID_STRIDE(c, d.n) {
+    if (d.flag[d.n + c]) {
+      const int32_t dm = d.dims[c];
+      if (dm >= 0 && dm < 32) bits |= 1u << dm;
+    }
+  }
+  for (int k = kWave / 2; k > 0; k >>= 1) bits |= __shfl_xor(bits, k, kWave);
+  if (lane_id() == 0 && bits) atomicOr(&d.ctr[cw::kNegDims], bits);
+}
+
 // Compression (kernel.h:495-520): rows of negative cells are removed from every column.
+// The rows of a dimension-d column are cells of dimension d-1, so only columns one dimension
+// above a dimension that holds a negative cell are scanned; the others keep their rows (the
+// removal only speeds up later additions, the pairs do not depend on it).
 __global__ void k_compress(Dev d) {
+  const uint32_t negdims = d.ctr[cw::kNegDims];
   GRID_STRIDE(c, d.n) {
     if (d.state[c] == ST_ZERO) continue;
     int32_t* col = d.pool + d.off[c];
     const int32_t l = d.len[c];
+    const int32_t dm = d.dims[c];
+    if (dm < 1 || dm > 32 || !((negdims >> (dm - 1)) & 1u)) {
+      if (d.state[c] == ST_PIVOT) d.pivmeta[col[l - 1]] = pivmeta_of(d.off[c], l, col);
+      continue;
+    }
     int32_t o = 0;
     for (int32_t k = 0; k < l; k++) {
       const int32_t r = col[k];
@@ -2062,6 +2085,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     else
       hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
     hipLaunchKernelGGL(k_lookup, dim3(g), dim3(256), 0, stream_, d, (const uint8_t*)hd, dual ? 1 : 0);
+    hipLaunchKernelGGL(k_negdims, dim3(g), dim3(256), 0, stream_, d);
     hipLaunchKernelGGL(k_clearing, dim3(g), dim3(256), 0, stream_, d);
     hipLaunchKernelGGL(k_compress, dim3(g), dim3(256), 0, stream_, d);
     HIPCHK(hipGetLastError());

@@ -1079,6 +1079,237 @@ __global__ void k_dual_dims(int64_t n, const int32_t* dims, int32_t maxd, const 
   }
 }
 
+__device__ void heapsort_i32(int32_t* a, int64_t n);
+
+// ---- anti-transpose by row windows ----------------------------------------------------
+// Dual column n-1-r lists n-1-c for every column c holding row r.  Rows are cut into windows
+// of kWinRows; (1) workgroups count the entries of each window over their chunk of columns
+// in an LDS histogram and add the totals with one global atomic per (chunk, window), (2) a
+// scan gives each window its run, (3) the same chunks place their entries into their
+// windows' runs in scratch (one reservation per chunk and window, ranks in LDS), (4) one
+// wavefront per window counts its rows, places and sorts each dual column in LDS and writes
+// the window's run out contiguously.  Block-diagonal batches keep a chunk's rows within a
+// few complexes, so the histograms absorb almost every entry.
+constexpr int kWinRows = 16;        // rows per window
+constexpr int kWinCap = 2048;       // entries per window (LDS per wavefront)
+constexpr int kHist = 8192;         // windows tracked in LDS by one bucketing workgroup
+constexpr int kChunkCols = 2048;    // columns per bucketing workgroup
+constexpr int kPlaceWaves = 4;
+
+__device__ __forceinline__ int32_t block_min_i32_256(int32_t v, int32_t* red) {
+  for (int k = kWave / 2; k > 0; k >>= 1) v = min(v, __shfl_xor(v, k, kWave));
+  __syncthreads();
+  if (lane_id() == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  int32_t m = red[0];
+  for (int k = 1; k < (int)(blockDim.x / kWave); k++) m = min(m, red[k]);
+  return m;
+}
+
+__global__ void __launch_bounds__(256) k_win_count(int64_t n, const int64_t* colptr, const int32_t* rows,
+                                                   const uint8_t* removed, uint32_t* wcnt) {
+  __shared__ uint32_t H[kHist];
+  __shared__ int32_t red[4];
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunkCols; c0 < n; c0 += (int64_t)gridDim.x * kChunkCols) {
+    const int64_t c1 = min(n, c0 + kChunkCols);
+    int32_t mn = 0x7fffffff;
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+      if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
+    const int32_t lo = block_min_i32_256(mn, red) / kWinRows;  // windows below lo get nothing
+    for (int i = threadIdx.x; i < kHist; i += blockDim.x) H[i] = 0;
+    __syncthreads();
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+      if (removed[c]) continue;
+      for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+        const int32_t w = rows[k] / kWinRows, d = w - lo;
+        if (d < kHist) atomicAdd(&H[d], 1u);
+        else atomicAdd(&wcnt[w], 1u);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHist; i += blockDim.x)
+      if (H[i]) atomicAdd(&wcnt[lo + i], H[i]);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_win_bucket(int64_t n, const int64_t* colptr, const int32_t* rows,
+                                                    const uint8_t* removed, const int64_t* woff, uint32_t* wfill,
+                                                    int32_t* scr_c, uint8_t* scr_r) {
+  __shared__ uint32_t H[kHist];
+  __shared__ uint32_t B[kHist];
+  __shared__ int32_t red[4];
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunkCols; c0 < n; c0 += (int64_t)gridDim.x * kChunkCols) {
+    const int64_t c1 = min(n, c0 + kChunkCols);
+    int32_t mn = 0x7fffffff;
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+      if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
+    const int32_t lo = block_min_i32_256(mn, red) / kWinRows;
+    for (int i = threadIdx.x; i < kHist; i += blockDim.x) H[i] = 0;
+    __syncthreads();
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+      if (removed[c]) continue;
+      for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+        const int32_t d = rows[k] / kWinRows - lo;
+        if (d < kHist) atomicAdd(&H[d], 1u);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHist; i += blockDim.x) {  // reserve this chunk's share of each window
+      const uint32_t h = H[i];
+      B[i] = h ? atomicAdd(&wfill[lo + i], h) : 0u;
+      H[i] = 0;
+    }
+    __syncthreads();
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+      if (removed[c]) continue;
+      for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+        const int32_t r = rows[k], w = r / kWinRows, d = w - lo;
+        const uint32_t pos = d < kHist ? B[d] + atomicAdd(&H[d], 1u) : atomicAdd(&wfill[w], 1u);
+        const int64_t idx = woff[w] + pos;
+        scr_c[idx] = (int32_t)c;
+        scr_r[idx] = (uint8_t)(r - w * kWinRows);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Bitonic sort of N values in registers (fully unrolled: static indices).
+template <int N>
+__device__ __forceinline__ void reg_bitonic(int32_t (&x)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const int32_t a = x[i], b = x[l];
+          const bool sw = up ? (a > b) : (a < b);
+          x[i] = sw ? b : a;
+          x[l] = sw ? a : b;
+        }
+      }
+}
+
+// Ascending sort of v[0:len) (len <= 256, in LDS) by one wavefront: 4 values per lane,
+// element i = 4*lane + t, bitonic across registers and lanes.
+__device__ __forceinline__ void wave_sort256(int32_t* v, int32_t len, int lane) {
+  int32_t x[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) x[t] = 4 * lane + t < len ? v[4 * lane + t] : 0x7fffffff;
+#pragma unroll
+  for (int k = 2; k <= 256; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 4) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int i = 4 * lane + t;
+          const int32_t y = __shfl_xor(x[t], j >> 2, kWave);
+          const bool up = (i & k) == 0, lower = (i & j) == 0;
+          x[t] = (lower == up) ? min(x[t], y) : max(x[t], y);
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int l = t ^ j;
+          if (l > t) {
+            const int i = 4 * lane + t;
+            const bool up = (i & k) == 0;
+            const int32_t a = x[t], b = x[l];
+            const bool sw = up ? (a > b) : (a < b);
+            x[t] = sw ? b : a;
+            x[l] = sw ? a : b;
+          }
+        }
+      }
+    }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+    if (4 * lane + t < len) v[4 * lane + t] = x[t];
+}
+
+__global__ void __launch_bounds__(kWave * kPlaceWaves) k_win_place(int64_t n, int64_t nw,
+                                                                   const int64_t* woff, const int32_t* scr_c,
+                                                                   const uint8_t* scr_r, int64_t* dcolptr,
+                                                                   int32_t* drows, uint32_t* overflow) {
+  __shared__ int32_t vb[kPlaceWaves][kWinCap];
+  __shared__ uint32_t cnt[kPlaceWaves][kWinRows], off[kPlaceWaves][kWinRows];
+  const int wv = threadIdx.x / kWave, lane = lane_id();
+  int32_t* v = vb[wv];
+  uint32_t* cn = cnt[wv];
+  uint32_t* of = off[wv];
+  const int64_t tot = woff[nw];  // entries of the dual (columns marked removed contribute none)
+  if (blockIdx.x == 0 && threadIdx.x == 0) dcolptr[n] = tot;
+  for (int64_t win = (int64_t)blockIdx.x * kPlaceWaves + wv; win < nw; win += (int64_t)gridDim.x * kPlaceWaves) {
+    const int64_t b = woff[win], e = woff[win + 1];
+    const int32_t T = (int32_t)(e - b);
+    const int64_t r0 = win * kWinRows;
+    const int32_t R = (int32_t)min((int64_t)kWinRows, n - r0);
+    if (T > kWinCap) {  // the caller falls back to the general anti-transpose
+      if (lane == 0) *overflow = 1u;
+      continue;
+    }
+    if (lane < kWinRows) cn[lane] = 0;
+    wave_sync();
+    for (int32_t q = lane; q < T; q += kWave) atomicAdd(&cn[scr_r[b + q]], 1u);
+    wave_sync();
+    if (lane == 0) {  // rows in decreasing order: the dual lists them by increasing n-1-r
+      uint32_t run = 0;
+      for (int32_t i = R - 1; i >= 0; i--) {
+        of[i] = run;
+        run += cn[i];
+      }
+    }
+    wave_sync();
+    const int64_t g0 = tot - e;
+    if (lane < R) dcolptr[n - 1 - (r0 + lane)] = g0 + of[lane];
+    wave_sync();
+    for (int32_t q = lane; q < T; q += kWave) {
+      const uint32_t pos = atomicAdd(&of[scr_r[b + q]], 1u);
+      v[pos] = (int32_t)(n - 1 - scr_c[b + q]);
+    }
+    wave_sync();
+    // each dual column ascending: short ones by their lane in registers, the others by the
+    // whole wavefront
+    const int32_t len = lane < R ? (int32_t)cn[lane] : 0;
+    const int32_t lo = lane < R ? (int32_t)of[lane] - len : 0;
+    if (len > 1 && len <= 16) {
+      int32_t x[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) x[i] = i < len ? v[lo + i] : 0x7fffffff;
+      reg_bitonic<16>(x);
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        if (i < len) v[lo + i] = x[i];
+    } else if (len > 16 && len <= 32) {
+      int32_t x[32];
+#pragma unroll
+      for (int i = 0; i < 32; i++) x[i] = i < len ? v[lo + i] : 0x7fffffff;
+      reg_bitonic<32>(x);
+#pragma unroll
+      for (int i = 0; i < 32; i++)
+        if (i < len) v[lo + i] = x[i];
+    }
+    wave_sync();
+    uint64_t longs = __ballot(len > 32);
+    while (longs) {
+      const int src = __ffsll((unsigned long long)longs) - 1;
+      longs &= longs - 1;
+      const int32_t L = __shfl(len, src, kWave), B0 = __shfl(lo, src, kWave);
+      if (L <= 256) wave_sort256(v + B0, L, lane);
+      else if (lane == 0) heapsort_i32(v + B0, L);
+      wave_sync();
+    }
+    for (int32_t q = lane; q < T; q += kWave) drows[g0 + q] = v[q];
+    wave_sync();
+  }
+}
+
 // Sort every segment v[off[s] : off[s+1]) ascending.  Segments longer than a lane's share
 // are sorted by a wavefront: bitonic in registers up to 64 entries, in LDS up to kSortLds,
 // heapsort by one lane beyond (rare).
@@ -1598,6 +1829,7 @@ class Engine {
     const char* tr = std::getenv("SSQ_TRACE_ROUNDS");  // per-round diagnostics on stderr
     trace_ = tr && tr[0] == '1';
     if (const char* e = std::getenv("SSQ_ROUNDS_PER_SYNC")) rounds_per_sync_ = std::max(1, std::min(8, std::atoi(e)));
+    if (const char* e = std::getenv("SSQ_TEST_NO_BLOCK_DUAL")) no_block_dual_ = e[0] == '1';
     // test hooks: smaller limits push short columns through the long-column paths
     if (const char* e = std::getenv("SSQ_TEST_SHORT_MAX")) short_max_ = std::max(2, std::min(kTile, std::atoi(e)));
     if (const char* e = std::getenv("SSQ_TEST_WINDOW_BITS"))
@@ -1638,7 +1870,8 @@ class Engine {
   void release() {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
                       &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
-                      &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_})
+                      &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_,
+                      &b_wcnt_, &b_woff_, &b_scrc_, &b_scrr_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
     if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
@@ -1676,7 +1909,9 @@ class Engine {
                    const int32_t* d_rows, const ssq_params& p, uint8_t** hd_out, ssq_stats* st);
   int scan_counts(const int32_t* cnt, int64_t n, int64_t* out);
   int build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
-                 const int32_t* dims, int64_t** dcolptr, int32_t** drows, int32_t** ddims);
+                 const int32_t* dims, bool any_removed, int64_t** dcolptr, int32_t** drows, int32_t** ddims);
+  int dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows, int64_t* dcolptr,
+                      int32_t* drows);
 
   static constexpr int kTotPending = 0;   // [0..7] pending columns per dimension
   static constexpr int kTotPairs = 8;     // pairs
@@ -1695,8 +1930,9 @@ class Engine {
   std::string err_;
   DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
       b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
-      b_drem_, b_scan_, b_pmeta_, b_prem_;
+      b_drem_, b_scan_, b_pmeta_, b_prem_, b_wcnt_, b_woff_, b_scrc_, b_scrr_;
   int64_t arena_hint_ = 0;
+  bool no_block_dual_ = false;  // test hook: always use the atomic anti-transpose
   bool timing_ = false;
   bool trace_ = false;
   int rounds_per_sync_ = 2;  // add rounds launched between two reads of the worklist counters
@@ -1942,8 +2178,41 @@ int Engine::scan_counts(const int32_t* cnt, int64_t n, int64_t* out) {
   return SSQ_OK;
 }
 
+// Anti-transpose by row windows (see k_win_place).  Returns 1 when it produced the dual, 0
+// when some window holds more than kWinCap entries (the general path is used then).
+int Engine::dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows,
+                            int64_t* dcolptr, int32_t* drows) {
+  const int64_t nw = (n + kWinRows - 1) / kWinRows;
+  uint32_t* wc;
+  int64_t* woff;
+  int32_t* scr_c;
+  uint8_t* scr_r;
+  int rc;
+  if ((rc = ensure(b_wcnt_, (size_t)(2 * nw + 16), &wc)) || (rc = ensure(b_woff_, (size_t)(nw + 1), &woff)) ||
+      (rc = ensure(b_scrc_, (size_t)std::max<int64_t>(nnz, 1), &scr_c)) ||
+      (rc = ensure(b_scrr_, (size_t)std::max<int64_t>(nnz, 1), &scr_r)))
+    return -rc;
+  uint32_t* wfill = wc + nw;
+  uint32_t* flag = wc + 2 * nw;
+  HIPCHK(hipMemsetAsync(wc, 0, sizeof(uint32_t) * (2 * nw + 16), stream_));
+  const int chunks = (int)std::min<int64_t>((n + kChunkCols - 1) / kChunkCols, (int64_t)cus_ * 8);
+  hipLaunchKernelGGL(k_win_count, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows,
+                     (const uint8_t*)d.removed, wc);
+  rc = scan_counts((const int32_t*)wc, nw, woff);
+  if (rc) return -rc;
+  hipLaunchKernelGGL(k_win_bucket, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows,
+                     (const uint8_t*)d.removed, (const int64_t*)woff, wfill, scr_c, scr_r);
+  const int64_t pg = std::min<int64_t>((nw + kPlaceWaves - 1) / kPlaceWaves, (int64_t)cus_ * 16);
+  hipLaunchKernelGGL(k_win_place, dim3((unsigned)std::max<int64_t>(1, pg)), dim3(kWave * kPlaceWaves), 0, stream_, n,
+                     nw, (const int64_t*)woff, (const int32_t*)scr_c, (const uint8_t*)scr_r, dcolptr, drows, flag);
+  HIPCHK(hipGetLastError());
+  rc = sync_read(flag, 1);
+  if (rc) return -rc;
+  return host_ctr_[0] ? 0 : 1;
+}
+
 int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
-                       const int32_t* dims, int64_t** dcolptr, int32_t** drows, int32_t** ddims) {
+                       const int32_t* dims, bool any_removed, int64_t** dcolptr, int32_t** drows, int32_t** ddims) {
   int32_t* cnt;
   uint8_t* drem;
   int rc;
@@ -1951,8 +2220,19 @@ int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int
       (rc = ensure(b_ddim_, (size_t)n, ddims)) || (rc = ensure(b_drem_, (size_t)n, &drem)) ||
       (rc = ensure(b_misc_, (size_t)n, &cnt)))
     return rc;
-  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
   const int g = grid_for(n);
+  int by_windows = 0;
+  if (n >= 1 && !no_block_dual_) {
+    by_windows = dual_by_windows(d, n, nnz, colptr, rows, *dcolptr, *drows);
+    if (by_windows < 0) return -by_windows;
+  }
+  if (by_windows) {
+    hipLaunchKernelGGL(k_dual_dims, dim3(g), dim3(256), 0, stream_, n, dims, maxdim, d.removed, *ddims, drem);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(d.removed, drem, (size_t)n, hipMemcpyDeviceToDevice, stream_));
+    return SSQ_OK;
+  }
+  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
   hipLaunchKernelGGL(k_dual_count, dim3(g), dim3(256), 0, stream_, n, colptr, rows, d.removed, cnt);
   rc = scan_counts(cnt, n, *dcolptr);
   if (rc) return rc;
@@ -2053,7 +2333,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     primal_rows = d_rows;
     if ((rc = ensure(b_prem_, (size_t)n, &primal_removed))) return rc;
     HIPCHK(hipMemcpyAsync(primal_removed, d.removed, (size_t)n, hipMemcpyDeviceToDevice, stream_));
-    rc = build_dual(d, n, nnz, maxdim, d_colptr, d_rows, d_dims, &dcp, &drw, &ddm);
+    rc = build_dual(d, n, nnz, maxdim, d_colptr, d_rows, d_dims, (p.model & 2) != 0, &dcp, &drw, &ddm);
     if (rc) return rc;
     if (p.model & 2) {  // removed cells drop their entries from the dual
       HIPCHK(hipMemcpyAsync(host_tot_ + kTotScratch, dcp + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));

@@ -281,3 +281,36 @@ def test_batch_reduce_points_equals_alone(h, maxdim):
             assert np.array_equal(pairs, ref) and np.array_equal(ess, ref_ess), (len(X), maxdim)
             mo = O.Matrix(m.colptr, m.rows, m.dims)
             assert np.array_equal(pairs, O.twist_pairs(mo))
+
+
+@pytest.mark.parametrize("general", [False, True])
+def test_dual_block_transpose_matches_general(monkeypatch, general):
+    """Cohomology through the per-block anti-transpose (independent complexes found by their
+    cut points) and through the general atomic one give the oracle's pairs."""
+    from scipy.spatial.distance import pdist
+    from specseqplusplus_amd import configs as CF
+    if general:
+        monkeypatch.setenv("SSQ_TEST_NO_BLOCK_DUAL", "1")
+    hd = S.Handle(0)
+    try:
+        rng = np.random.default_rng(77)
+        mats = []
+        for i, n in enumerate([3, 60, 1, 140, 0, 90, 200, 17, 110]):
+            X = CF.cloud(rng, i % 3, n).astype(np.float32) if n else np.zeros((0, 3), np.float32)
+            r = float(np.percentile(pdist(X.astype(np.float64)), 30)) if n > 1 else 0.5
+            mats.append(S.vr_build(X, r, 2))
+        p = S.params(dualize=1, ss_block_size=1 << 30)
+        outs = hd.batch_reduce(mats, p)
+        for m, (pairs, ess) in zip(mats, outs):
+            mo = O.Matrix(m.colptr, m.rows, m.dims)
+            assert np.array_equal(pairs, O.twist_pairs(mo)), (general, m.n)
+        # the same batch resident in HBM (the path of a device-resident caller)
+        from specseqplusplus_amd.autotune import block_diagonal
+        bd, base = block_diagonal(mats)
+        dm = hd.upload(bd)
+        out = hd.device_empty(2 * bd.n, np.int64)
+        npairs, _ = hd.reduce_resident(dm, out, p)
+        allp = out.numpy(2 * npairs).reshape(-1, 2)
+        assert np.array_equal(allp, O.twist_pairs(O.Matrix(bd.colptr, bd.rows, bd.dims)))
+    finally:
+        hd.close()

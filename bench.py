@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--single", choices=["c1", "c2", "c3"],
                     help="time one single-complex configuration instead of the batched stream")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="concurrent groups (one HIP stream each) the batched stream is split into")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default batched parameters instead of tuning each size class")
     args = ap.parse_args()
@@ -257,35 +259,46 @@ def main():
     split = int(np.sqrt(args.nmin * args.nmax))
     classes = [np.where(ns < split)[0], np.where(ns >= split)[0]]
     classes = [c for c in classes if len(c)]
+    # each size class is split into interleaved groups; every group is one block-diagonal
+    # batch on its own handle (HIP stream), and a step reduces all groups concurrently
+    total_cells = sum(m.n for m in mats)
     plans = []
+    class_params = []
     for ci, idx in enumerate(classes):
         if args.no_tune:
             chosen, table = dict(BATCH_PARAMS), []
         else:
             chosen, table = tune(h, [mats[i] for i in idx[:min(len(idx), 96)]])
-        bd, cbase = bdiag([mats[i] for i in idx])
-        plans.append({"idx": idx, "dm": h.upload(bd), "out": h.device_empty(2 * bd.n, np.int64),
-                      "p": S.params(**chosen), "chosen": chosen, "table": table, "base": cbase,
-                      "cells": bd.n, "nnz": bd.nnz})
-        print(f"[bench] class {ci}: {len(idx)} complexes, {bd.n} cells, tuned {chosen} "
+        class_params.append((chosen, table))
+        cells = sum(mats[i].n for i in idx)
+        parts = max(1, int(round(args.streams * cells / total_cells)))
+        for g in range(parts):
+            gidx = idx[g::parts]
+            hg = S.Handle(0)
+            bd, cbase = bdiag([mats[i] for i in gidx])
+            plans.append({"idx": gidx, "h": hg, "dm": hg.upload(bd), "out": hg.device_empty(2 * bd.n, np.int64),
+                          "p": S.params(**chosen), "chosen": chosen, "table": table, "base": cbase,
+                          "cells": bd.n, "nnz": bd.nnz, "cls": ci})
+        print(f"[bench] class {ci}: {len(idx)} complexes, {cells} cells in {parts} group(s), tuned {chosen} "
               f"({[t['ms'] for t in table]} ms)", file=sys.stderr, flush=True)
     del bd
-    h.synchronize()
+    for pl in plans:
+        pl["h"].synchronize()
+    pool = ThreadPoolExecutor(len(plans))
 
     def step():
-        sts = []
+        futs = [pool.submit(pl["h"].reduce_resident, pl["dm"], pl["out"], pl["p"]) for pl in plans]
+        res = [f.result() for f in futs]
         for pl in plans:
-            sts.append(h.reduce_resident(pl["dm"], pl["out"], pl["p"]))
-        return sts
+            pl["h"].synchronize()
+        return res
 
     for _ in range(args.warmup):
         step()
-    h.synchronize()
     stats = []
     t = time.perf_counter()
     for _ in range(args.steps):
         stats.extend(st for _, st in step())
-    h.synchronize()
     dt = time.perf_counter() - t
     ms_per_step = dt * 1e3 / args.steps
     diagrams_per_s = args.complexes * args.steps / dt
@@ -296,8 +309,8 @@ def main():
         for k, arr in enumerate(split_pairs(pp, pl["base"])):
             per[pl["idx"][k]] = arr
     class_of = np.zeros(args.complexes, int)
-    for ci, pl in enumerate(plans):
-        class_of[pl["idx"]] = ci
+    for pl in plans:
+        class_of[pl["idx"]] = pl["cls"]
 
     # parity sample against the oracle (CPU); also the CPU baseline rate
     parity = {"checked": 0, "bit_exact": True}
@@ -350,7 +363,8 @@ def main():
     plain = None
     if args.plain > 0:
         idx = np.random.default_rng(2).permutation(args.complexes)[:args.plain]
-        dev_mats = [(h.upload(mats[i]), plans[class_of[i]]["p"]) for i in idx]
+        cls_p = [S.params(**c) for c, _ in class_params]
+        dev_mats = [(h.upload(mats[i]), cls_p[class_of[i]]) for i in idx]
         d_pairs = h.device_empty(2 * max(mats[i].n for i in idx), np.int64)
         big = int(np.argmax([mats[i].n for i in idx]))  # size the engine's buffers first
         h.reduce_resident(dev_mats[big][0], d_pairs, dev_mats[big][1])
@@ -440,9 +454,10 @@ def main():
         "config": {"workload": f"batched: {args.complexes} Rips complexes, n log-uniform [{args.nmin},{args.nmax}], "
                                f"cube/circle/torus, r = 30th percentile, dim<=2, block-diagonal in HBM",
                    "cells": int(N), "nnz": int(NNZ),
-                   "size_classes": [{"complexes": int(len(pl["idx"])), "cells": int(pl["cells"]), "nnz": int(pl["nnz"]),
-                                     "params": pl["chosen"], "tuning_ms": [t["ms"] for t in pl["table"]]}
-                                    for pl in plans]},
+                   "streams": args.streams,
+                   "groups": [{"size_class": pl["cls"], "complexes": int(len(pl["idx"])), "cells": int(pl["cells"]),
+                               "nnz": int(pl["nnz"]), "params": pl["chosen"],
+                               "tuning_ms": [t["ms"] for t in pl["table"]]} for pl in plans]},
         "plain_gpu_diagrams_per_s": round(plain, 2) if plain else None,
         "batched_over_plain": round(diagrams_per_s / plain, 2) if plain else None,
         "served": served, "parity": parity,

@@ -211,8 +211,10 @@ __global__ void k_leftone(Dev d) {
 // column is positive (its column reduces to zero; flag[row] = 1); a column that is leftmost
 // in its low row is an apparent pair; a column leftmost in any row is negative
 // (flag[n + col] = 1).  The two flags live in separate bytes: plain stores of 1 only.
+// Also collects the dimensions that hold a negative cell (one bit each, for compression).
 __global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
   uint32_t apparent = 0;  // summed per wavefront: one atomic per wave, not per pair
+  uint32_t negdims = 0;
   GRID_STRIDE(c, d.n) {
     const int32_t l = d.len[c];
     if (l == 0) continue;
@@ -223,16 +225,30 @@ __global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
       d.state[c] = ST_PIVOT;
       apparent++;
     }
-    if (neg_known) continue;
-    const int32_t* col = d.pool + d.off[c];
-    for (int32_t k = 0; k < l; k++)
-      if (d.leftone[col[k]] == (uint32_t)c) {
-        d.flag[d.n + c] = 1;
-        break;
-      }
+    bool neg;
+    if (neg_known) {
+      neg = d.flag[d.n + c] != 0;
+    } else {
+      neg = false;
+      const int32_t* col = d.pool + d.off[c];
+      for (int32_t k = 0; k < l; k++)
+        if (d.leftone[col[k]] == (uint32_t)c) {
+          d.flag[d.n + c] = 1;
+          neg = true;
+          break;
+        }
+    }
+    if (neg) {
+      const int32_t dm = d.dims[c];
+      if (dm >= 0 && dm < 32) negdims |= 1u << dm;
+    }
   }
-  for (int k = kWave / 2; k > 0; k >>= 1) apparent += __shfl_xor(apparent, k, kWave);
+  for (int k = kWave / 2; k > 0; k >>= 1) {
+    apparent += __shfl_xor(apparent, k, kWave);
+    negdims |= __shfl_xor(negdims, k, kWave);
+  }
   if (lane_id() == 0 && apparent) atomicAdd(&d.ctr[cw::kApparent], apparent);
+  if (lane_id() == 0 && negdims) atomicOr(&d.ctr[cw::kNegDims], negdims);
   if (hdclear)
     GRID_STRIDE(c, d.n) if (hdclear[c]) d.flag[c] = 1;
 }
@@ -252,54 +268,73 @@ __global__ void k_dual_init_from_primal(int64_t n, const int64_t* colptr, const 
   }
 }
 
-// Clearing (kernel.h:483-493): positive columns that are not pivots are zeroed.
-__global__ void k_clearing(Dev d) {
+
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+  for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+  for (int d = kWave / 2; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+// Clearing, compression and the per-dimension row bounds of the page schedule in one pass
+// over the columns (a column's steps read only what k_lookup left and the column itself):
+//  * clearing (kernel.h:483-493): flagged columns that are not pivots are zeroed;
+//  * compression (kernel.h:495-520): rows of negative cells are removed.  The rows of a
+//    dimension-d column are cells of dimension d-1, so only columns one dimension above a
+//    dimension holding a negative cell are scanned; the others keep their rows (the removal
+//    only speeds up later additions, the pairs do not depend on it);
+//  * bounds: per dimension, the largest pending low + 1 and the smallest row.
+__global__ void k_clear_compress_stats(Dev d, int32_t* maxrow, int32_t* minrow) {
+  const uint32_t negdims = d.ctr[cw::kNegDims];
+  int32_t mx[kMaxDims], mn[kMaxDims];
+  for (int k = 0; k < kMaxDims; k++) {
+    mx[k] = -1;
+    mn[k] = 0x7fffffff;
+  }
   GRID_STRIDE(c, d.n) {
-    if (d.flag[c] && d.state[c] != ST_PIVOT) {
+    uint8_t st = d.state[c];
+    if (st == ST_ZERO) continue;
+    if (d.flag[c] && st != ST_PIVOT) {  // clearing (kernel.h:483-493)
       d.state[c] = ST_ZERO;
       d.len[c] = 0;
       d.low[c] = -1;
-    }
-  }
-}
-
-// Dimensions that hold a negative cell (one bit each), for k_compress.
-__global__ void k_negdims(Dev d) {
-  uint32_t bits = 0;
-  GRID_STRIDE(c, d.n) {
-    if (d.flag[d.n + c]) {
-      const int32_t dm = d.dims[c];
-      if (dm >= 0 && dm < 32) bits |= 1u << dm;
-    }
-  }
-  for (int k = kWave / 2; k > 0; k >>= 1) bits |= __shfl_xor(bits, k, kWave);
-  if (lane_id() == 0 && bits) atomicOr(&d.ctr[cw::kNegDims], bits);
-}
-
-// Compression (kernel.h:495-520): rows of negative cells are removed from every column.
-// The rows of a dimension-d column are cells of dimension d-1, so only columns one dimension
-// above a dimension that holds a negative cell are scanned; the others keep their rows (the
-// removal only speeds up later additions, the pairs do not depend on it).
-__global__ void k_compress(Dev d) {
-  const uint32_t negdims = d.ctr[cw::kNegDims];
-  GRID_STRIDE(c, d.n) {
-    if (d.state[c] == ST_ZERO) continue;
-    int32_t* col = d.pool + d.off[c];
-    const int32_t l = d.len[c];
-    const int32_t dm = d.dims[c];
-    if (dm < 1 || dm > 32 || !((negdims >> (dm - 1)) & 1u)) {
-      if (d.state[c] == ST_PIVOT) d.pivmeta[col[l - 1]] = pivmeta_of(d.off[c], l, col);
       continue;
     }
-    int32_t o = 0;
-    for (int32_t k = 0; k < l; k++) {
-      const int32_t r = col[k];
-      if (!d.flag[d.n + r]) col[o++] = r;
+    int32_t* col = d.pool + d.off[c];
+    int32_t l = d.len[c];
+    const int32_t dm = d.dims[c];
+    if (dm >= 1 && dm <= 32 && ((negdims >> (dm - 1)) & 1u)) {  // compression (kernel.h:495-520)
+      int32_t o = 0;
+      for (int32_t k = 0; k < l; k++) {
+        const int32_t r = col[k];
+        if (!d.flag[d.n + r]) col[o++] = r;
+      }
+      l = o;
+      d.len[c] = o;
+      d.low[c] = o ? col[o - 1] : -1;
+      if (!o) {
+        d.state[c] = ST_ZERO;
+        continue;
+      }
     }
-    d.len[c] = o;
-    d.low[c] = o ? col[o - 1] : -1;
-    if (!o) d.state[c] = ST_ZERO;
-    else if (d.state[c] == ST_PIVOT) d.pivmeta[col[o - 1]] = pivmeta_of(d.off[c], o, col);
+    if (st == ST_PIVOT) d.pivmeta[col[l - 1]] = pivmeta_of(d.off[c], l, col);
+    if (dm >= 0 && dm < kMaxDims) {
+      const int32_t first = col[0], lowp1 = col[l - 1] + 1;
+      for (int k = 0; k < kMaxDims; k++)
+        if (k == dm) {  // static register indexing
+          if (st == ST_PENDING) mx[k] = max(mx[k], lowp1);
+          mn[k] = min(mn[k], first);
+        }
+    }
+  }
+  for (int k = 0; k < kMaxDims; k++) {
+    const int32_t a = wave_max(mx[k]), b = wave_min(mn[k]);
+    if (lane_id() == 0) {
+      if (a >= 0) atomicMax(&maxrow[k], a);
+      if (b != 0x7fffffff) atomicMin(&minrow[k], b);
+    }
   }
 }
 
@@ -318,14 +353,6 @@ __global__ void k_pivmeta(Dev d, int r) {
   }
 }
 
-__device__ __forceinline__ int32_t wave_max(int32_t v) {
-  for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
-  return v;
-}
-__device__ __forceinline__ int32_t wave_min(int32_t v) {
-  for (int d = kWave / 2; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d, kWave));
-  return v;
-}
 
 // one atomic per wavefront (a plain per-thread atomic on one address serialises millions)
 __global__ void k_maxdim(int64_t n, const int32_t* dims, int32_t* out, uint32_t* bad) {
@@ -346,33 +373,6 @@ __global__ void k_maxdim(int64_t n, const int32_t* dims, int32_t* out, uint32_t*
   }
 }
 
-// Per-dimension page bounds: max low over pending columns (+1) and min first row over
-// non-zero columns (specseq.h:365-409).
-__global__ void k_dimstats(Dev d, int32_t* maxrow, int32_t* minrow) {
-  int32_t mx[kMaxDims], mn[kMaxDims];
-  for (int k = 0; k < kMaxDims; k++) {
-    mx[k] = -1;
-    mn[k] = 0x7fffffff;
-  }
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < d.n; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = base + threadIdx.x;
-    if (c >= d.n) continue;
-    const int32_t dm = d.dims[c];
-    if (dm < 0 || dm >= kMaxDims || d.len[c] == 0) continue;
-    for (int k = 0; k < kMaxDims; k++)
-      if (k == dm) {  // static register indexing
-        if (d.state[c] == ST_PENDING) mx[k] = max(mx[k], d.low[c] + 1);
-        mn[k] = min(mn[k], d.pool[d.off[c]]);
-      }
-  }
-  for (int k = 0; k < kMaxDims; k++) {
-    const int32_t a = wave_max(mx[k]), b = wave_min(mn[k]);
-    if (lane_id() == 0) {
-      if (a >= 0) atomicMax(&maxrow[k], a);
-      if (b != 0x7fffffff) atomicMin(&minrow[k], b);
-    }
-  }
-}
 
 // ------------------------------------------------------------------------------------
 // ordered compaction by a small key (stable partition), used for the per-dimension
@@ -2365,18 +2365,16 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     else
       hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
     hipLaunchKernelGGL(k_lookup, dim3(g), dim3(256), 0, stream_, d, (const uint8_t*)hd, dual ? 1 : 0);
-    hipLaunchKernelGGL(k_negdims, dim3(g), dim3(256), 0, stream_, d);
-    hipLaunchKernelGGL(k_clearing, dim3(g), dim3(256), 0, stream_, d);
-    hipLaunchKernelGGL(k_compress, dim3(g), dim3(256), 0, stream_, d);
+    int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
+    hipLaunchKernelGGL(k_rowstat_init, dim3(1), dim3(64), 0, stream_, rowstat);
+    hipLaunchKernelGGL(k_clear_compress_stats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
     HIPCHK(hipGetLastError());
     // pending lists per dimension and page bounds, read back in one round trip
     const int pend_keys = std::min(kMaxDims, std::max(maxdim, 1) + 1);  // dims 1..maxdim hold pending columns
     HIPCHK(hipMemsetAsync(tot_ + kTotPending, 0, sizeof(int64_t) * kMaxDims, stream_));
     rc = compact(n, pend_keys, KeyPendingDim{d.state, d.dims, pend_keys}, EmitIndex{U}, nullptr, kTotPending);
     if (rc) return rc;
-    int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
-    hipLaunchKernelGGL(k_rowstat_init, dim3(1), dim3(64), 0, stream_, rowstat);
-    hipLaunchKernelGGL(k_dimstats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
+
     rc = sync_all();
     if (rc) return rc;
     int64_t cnt[kMaxDims];

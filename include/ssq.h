@@ -152,6 +152,12 @@ int ssq_server_poll(ssq_server* s, int64_t ticket, ssq_pairs* out, double* laten
 int ssq_server_wait_all(ssq_server* s);
 int ssq_server_destroy(ssq_server* s);
 
+/* Host edge collapse (model bit 1): kept[i] = 1 for every edge that GUDHI's sequential
+ * collapser keeps, edges given in filtration order (edge i has filtration i) by their two
+ * vertex labels.  Replaces flag_complex_collapse_edges as called by specseq.h:410-424
+ * (Flag_complex_edge_collapser.h:503-524).  threads <= 0: all host threads.  No GPU. */
+int ssq_collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kept, int threads);
+
 /* ---- Inputs (phat-highclear/addons/rips.cpp:49-87 semantics) ------------------------- */
 /* Vietoris-Rips boundary matrix of npts points in R^pdim: edges with float32 distance
  * <= r, cliques up to dimension maxdim, ordered by (filtration, dim, lexicographic vertex

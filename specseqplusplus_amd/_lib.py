@@ -85,6 +85,8 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
     "ssq_vr_build_gpu": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
+    "ssq_collapse_edges": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]),
     "ssq_batch_reduce_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),
                                                ctypes.POINTER(ctypes.c_int64), ctypes.c_int,

@@ -83,6 +83,17 @@ def vr_build(points, r, maxdim):
     return out
 
 
+def collapse_edges(u, v, threads=0):
+    """Kept-edge mask of the flag-complex edge collapse (edges in filtration order)."""
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    kept = np.zeros(len(u), dtype=np.uint8)
+    L.check(L.lib().ssq_collapse_edges(len(u), u.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                       v.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                       kept.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), int(threads)))
+    return kept
+
+
 class Handle:
     """One engine (HIP stream + device state) on one GPU."""
 

@@ -29,7 +29,35 @@ struct Graph {
   std::vector<int64_t> off;
   std::vector<int32_t> nbr, eid;
   std::vector<uint8_t> crit;
+  // Per edge e = (a, b): its common neighbours x in the whole graph, ascending, with the ids
+  // of (a, x) and (b, x) -- the edge's triangles.  The valid common neighbours of a test are
+  // a filter of this list, not a merge of two adjacency lists.  Empty when too large.
+  std::vector<int64_t> toff;
+  std::vector<int32_t> tx, tea, teb;
+  std::vector<int32_t> eu, ev;
 };
+constexpr int64_t kTriListMax = int64_t(1) << 28;  // entries (12 bytes each)
+
+template <typename Valid>
+int64_t common(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* out, int32_t* ea,
+               int32_t* eb);
+
+// Valid common neighbours of edge e (ascending), with the two edge ids when ea/eb are given.
+template <typename Valid>
+int64_t common_e(const Graph& g, int32_t e, Valid valid, int32_t* out, int32_t* ea, int32_t* eb) {
+  if (g.toff.empty()) return common(g, g.eu[e], g.ev[e], valid, out, ea, eb);
+  int64_t k = 0;
+  for (int64_t t = g.toff[e], te = g.toff[e + 1]; t < te; t++) {
+    const int32_t e1 = g.tea[t], e2 = g.teb[t];
+    if (!valid(e1) || !valid(e2)) continue;
+    if (ea) {
+      ea[k] = e1;
+      eb[k] = e2;
+    }
+    out[k++] = g.tx[t];
+  }
+  return k;
+}
 
 template <typename Valid>
 int64_t common(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* out, int32_t* ea,
@@ -43,7 +71,8 @@ int64_t common(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* out, 
     else if (x > y) j++;
     else {
       if (ea) { ea[k] = g.eid[i]; eb[k] = g.eid[j]; }
-      out[k++] = x;
+      if (out) out[k] = x;
+      k++;
       i++;
       j++;
     }
@@ -68,8 +97,8 @@ inline bool covers(const Graph& g, int32_t w, const int32_t* I, int64_t m, Valid
 // first and receives the dominating vertex found (the answer does not depend on the order
 // in which candidates are tried).
 template <typename Valid>
-bool dominated(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* I, int32_t* witness) {
-  const int64_t m = common(g, a, b, valid, I, nullptr, nullptr);
+bool dominated(const Graph& g, int32_t e, Valid valid, int32_t* I, int32_t* witness) {
+  const int64_t m = common_e(g, e, valid, I, nullptr, nullptr);
   if (m <= 1) return m == 1;
   const int32_t hint = *witness;
   if (hint >= 0 && std::binary_search(I, I + m, hint) && covers(g, hint, I, m, valid)) return true;
@@ -124,12 +153,48 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
   }
   int64_t maxdeg = 1;
   for (int64_t a = 0; a < nv; a++) maxdeg = std::max(maxdeg, g.off[a + 1] - g.off[a]);
+  g.eu = eu;
+  g.ev = ev;
+  if (threads < 1) threads = 1;
+  auto parallel_edges = [&](auto&& body) {  // body(e) for every edge, on all threads
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+      for (;;) {
+        const int64_t s0 = next.fetch_add(1024);
+        if (s0 >= ne) break;
+        for (int64_t e = s0; e < std::min(ne, s0 + 1024); e++) body(e);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+  };
+  {  // triangle lists of every edge
+    std::vector<int64_t> cnt((size_t)ne + 1, 0);
+    parallel_edges([&](int64_t e) {
+      cnt[e + 1] = common(g, eu[e], ev[e], [](int32_t) { return true; }, nullptr, nullptr, nullptr);
+    });
+    for (int64_t e = 0; e < ne; e++) cnt[e + 1] += cnt[e];
+    if (cnt[ne] <= kTriListMax) {
+      g.tx.resize((size_t)cnt[ne]);
+      g.tea.resize((size_t)cnt[ne]);
+      g.teb.resize((size_t)cnt[ne]);
+      parallel_edges([&](int64_t e) {
+        const int64_t o = cnt[e];
+        int32_t* X = g.tx.data() + o;
+        int32_t* A = g.tea.data() + o;
+        int32_t* B = g.teb.data() + o;
+        common(g, eu[e], ev[e], [](int32_t) { return true; }, X, A, B);
+      });
+      g.toff.swap(cnt);
+    }
+  }
 
   // 1. main-loop decisions, in parallel
   std::vector<uint8_t> main_crit((size_t)ne, 0);
   std::vector<int32_t> witness((size_t)ne, -1);
   main_crit[0] = 1;
-  if (threads < 1) threads = 1;
   {
     std::atomic<int64_t> next{1};
     auto work = [&]() {
@@ -140,7 +205,7 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
         int64_t e = std::min(ne, s + 256);
         for (int64_t i = s; i < e; i++) {
           int32_t idx = (int32_t)i;
-          main_crit[i] = !dominated(g, eu[i], ev[i], [idx](int32_t x) { return x < idx; }, I.data(), &witness[i]);
+          main_crit[i] = !dominated(g, idx, [idx](int32_t x) { return x < idx; }, I.data(), &witness[i]);
         }
       }
     };
@@ -155,10 +220,10 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
   std::vector<int32_t> I((size_t)maxdeg), ea((size_t)maxdeg), eb((size_t)maxdeg);
   std::vector<int32_t> stamp((size_t)ne, -1);
   std::priority_queue<int32_t> heap;
-  auto three_clique = [&](int32_t thresh, int32_t a, int32_t b, int32_t sid) {
+  auto three_clique = [&](int32_t thresh, int32_t e, int32_t sid) {
     const uint8_t* crit = g.crit.data();
-    int64_t m = common(g, a, b, [thresh, crit](int32_t x) { return x < thresh || crit[x]; }, I.data(),
-                       ea.data(), eb.data());
+    int64_t m = common_e(g, e, [thresh, crit](int32_t x) { return x < thresh || crit[x]; }, I.data(), ea.data(),
+                         eb.data());
     for (int64_t k = 0; k < m; k++) {
       int32_t e1 = ea[k], e2 = eb[k];
       if (!crit[e1] && e1 < thresh && stamp[e1] != sid) { stamp[e1] = sid; heap.push(e1); }
@@ -172,20 +237,26 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
     if (!main_crit[idx]) continue;
     g.crit[idx] = 1;
     kept[idx] = 1;
-    three_clique(idx, eu[idx], ev[idx], idx);
+    three_clique(idx, idx, idx);
     while (!heap.empty()) {
       int32_t cur = heap.top();
       heap.pop();
       if (g.crit[cur]) continue;
       const uint8_t* crit = g.crit.data();
-      if (dominated(g, eu[cur], ev[cur], [cur, crit](int32_t x) { return x < cur || crit[x]; }, I.data(),
-                    &witness[cur]))
+      if (dominated(g, cur, [cur, crit](int32_t x) { return x < cur || crit[x]; }, I.data(), &witness[cur]))
         continue;
       g.crit[cur] = 1;
       kept[cur] = 1;
-      three_clique(cur, eu[cur], ev[cur], idx);
+      three_clique(cur, cur, idx);
     }
   }
 }
 
 }  // namespace ssq
+
+extern "C" int ssq_collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kept, int threads) {
+  if (ne < 0 || (ne > 0 && (!u || !v || !kept))) return SSQ_EINVAL;
+  if (threads <= 0) threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  ssq::collapse_edges(ne, u, v, kept, threads);
+  return SSQ_OK;
+}

@@ -98,3 +98,24 @@ def test_cli_help_and_argument_errors(tmp_path):
     args[4] = "ascii"
     args[1] = str(tmp_path / "missing.txt")
     assert subprocess.run(args, capture_output=True).returncode == 2
+
+
+def test_host_collapse_matches_fixtures_and_oracle():
+    """The library's parallel host collapse keeps exactly GUDHI's edges: the committed
+    kept-edge fixtures, the oracle's sequential restatement, and (where it is built here)
+    the reference collapser itself, on graphs dense enough for long cascades."""
+    import fixtures as F
+    for u, v, kept in F.kept_edge_cases():
+        for threads in (1, 4):
+            assert np.array_equal(S.collapse_edges(u, v, threads), kept)
+    from oracle import oracle as O
+    import complexes as C
+    for seed in range(6):
+        X = C.uniform_points(60 + 25 * seed, 3, 700 + seed)
+        m = C.vr_matrix(X, 0.35 + 0.02 * seed, 1)
+        e = np.where(m.dims == 1)[0]
+        u, v = m.rows[m.colptr[e]], m.rows[m.colptr[e] + 1]
+        got = S.collapse_edges(u, v, 3)
+        assert np.array_equal(got, O.collapse_edges(u, v)), seed
+        if O.ref_available():
+            assert np.array_equal(got, O.ref_collapse_edges(u, v)), seed

@@ -1112,15 +1112,28 @@ __global__ void __launch_bounds__(256) k_win_count(int64_t n, const int64_t* col
   __shared__ int32_t red[4];
   for (int64_t c0 = (int64_t)blockIdx.x * kChunkCols; c0 < n; c0 += (int64_t)gridDim.x * kChunkCols) {
     const int64_t c1 = min(n, c0 + kChunkCols);
+    const int64_t k0 = colptr[c0], k1 = colptr[c1];
     int32_t mn = 0x7fffffff;
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
-      if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
+    if (removed) {
+      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+        if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
+    } else {  // no column is removed: the chunk's entries are one contiguous run
+      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) mn = min(mn, rows[k]);
+    }
     const int32_t lo = block_min_i32_256(mn, red) / kWinRows;  // windows below lo get nothing
     for (int i = threadIdx.x; i < kHist; i += blockDim.x) H[i] = 0;
     __syncthreads();
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-      if (removed[c]) continue;
-      for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+    if (removed) {
+      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        if (removed[c]) continue;
+        for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+          const int32_t w = rows[k] / kWinRows, d = w - lo;
+          if (d < kHist) atomicAdd(&H[d], 1u);
+          else atomicAdd(&wcnt[w], 1u);
+        }
+      }
+    } else {
+      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         const int32_t w = rows[k] / kWinRows, d = w - lo;
         if (d < kHist) atomicAdd(&H[d], 1u);
         else atomicAdd(&wcnt[w], 1u);
@@ -1141,15 +1154,27 @@ __global__ void __launch_bounds__(256) k_win_bucket(int64_t n, const int64_t* co
   __shared__ int32_t red[4];
   for (int64_t c0 = (int64_t)blockIdx.x * kChunkCols; c0 < n; c0 += (int64_t)gridDim.x * kChunkCols) {
     const int64_t c1 = min(n, c0 + kChunkCols);
+    const int64_t k0 = colptr[c0], k1 = colptr[c1];
     int32_t mn = 0x7fffffff;
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
-      if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
+    if (removed) {
+      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+        if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
+    } else {
+      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) mn = min(mn, rows[k]);
+    }
     const int32_t lo = block_min_i32_256(mn, red) / kWinRows;
     for (int i = threadIdx.x; i < kHist; i += blockDim.x) H[i] = 0;
     __syncthreads();
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-      if (removed[c]) continue;
-      for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+    if (removed) {
+      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        if (removed[c]) continue;
+        for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+          const int32_t d = rows[k] / kWinRows - lo;
+          if (d < kHist) atomicAdd(&H[d], 1u);
+        }
+      }
+    } else {
+      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         const int32_t d = rows[k] / kWinRows - lo;
         if (d < kHist) atomicAdd(&H[d], 1u);
       }
@@ -1162,7 +1187,7 @@ __global__ void __launch_bounds__(256) k_win_bucket(int64_t n, const int64_t* co
     }
     __syncthreads();
     for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-      if (removed[c]) continue;
+      if (removed && removed[c]) continue;
       for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
         const int32_t r = rows[k], w = r / kWinRows, d = w - lo;
         const uint32_t pos = d < kHist ? B[d] + atomicAdd(&H[d], 1u) : atomicAdd(&wfill[w], 1u);
@@ -1910,8 +1935,8 @@ class Engine {
   int scan_counts(const int32_t* cnt, int64_t n, int64_t* out);
   int build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
                  const int32_t* dims, bool any_removed, int64_t** dcolptr, int32_t** drows, int32_t** ddims);
-  int dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows, int64_t* dcolptr,
-                      int32_t* drows);
+  int dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows, bool any_removed,
+                      int64_t* dcolptr, int32_t* drows);
 
   static constexpr int kTotPending = 0;   // [0..7] pending columns per dimension
   static constexpr int kTotPairs = 8;     // pairs
@@ -2181,7 +2206,8 @@ int Engine::scan_counts(const int32_t* cnt, int64_t n, int64_t* out) {
 // Anti-transpose by row windows (see k_win_place).  Returns 1 when it produced the dual, 0
 // when some window holds more than kWinCap entries (the general path is used then).
 int Engine::dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows,
-                            int64_t* dcolptr, int32_t* drows) {
+                            bool any_removed, int64_t* dcolptr, int32_t* drows) {
+  const uint8_t* rem = any_removed ? (const uint8_t*)d.removed : nullptr;
   const int64_t nw = (n + kWinRows - 1) / kWinRows;
   uint32_t* wc;
   int64_t* woff;
@@ -2196,12 +2222,12 @@ int Engine::dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colpt
   uint32_t* flag = wc + 2 * nw;
   HIPCHK(hipMemsetAsync(wc, 0, sizeof(uint32_t) * (2 * nw + 16), stream_));
   const int chunks = (int)std::min<int64_t>((n + kChunkCols - 1) / kChunkCols, (int64_t)cus_ * 8);
-  hipLaunchKernelGGL(k_win_count, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows,
-                     (const uint8_t*)d.removed, wc);
+  hipLaunchKernelGGL(k_win_count, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows, rem,
+                     wc);
   rc = scan_counts((const int32_t*)wc, nw, woff);
   if (rc) return -rc;
-  hipLaunchKernelGGL(k_win_bucket, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows,
-                     (const uint8_t*)d.removed, (const int64_t*)woff, wfill, scr_c, scr_r);
+  hipLaunchKernelGGL(k_win_bucket, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows, rem,
+                     (const int64_t*)woff, wfill, scr_c, scr_r);
   const int64_t pg = std::min<int64_t>((nw + kPlaceWaves - 1) / kPlaceWaves, (int64_t)cus_ * 16);
   hipLaunchKernelGGL(k_win_place, dim3((unsigned)std::max<int64_t>(1, pg)), dim3(kWave * kPlaceWaves), 0, stream_, n,
                      nw, (const int64_t*)woff, (const int32_t*)scr_c, (const uint8_t*)scr_r, dcolptr, drows, flag);
@@ -2223,7 +2249,7 @@ int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int
   const int g = grid_for(n);
   int by_windows = 0;
   if (n >= 1 && !no_block_dual_) {
-    by_windows = dual_by_windows(d, n, nnz, colptr, rows, *dcolptr, *drows);
+    by_windows = dual_by_windows(d, n, nnz, colptr, rows, any_removed, *dcolptr, *drows);
     if (by_windows < 0) return -by_windows;
   }
   if (by_windows) {

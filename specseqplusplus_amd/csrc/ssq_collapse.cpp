@@ -33,7 +33,7 @@ struct Graph {
   // of (a, x) and (b, x) -- the edge's triangles.  The valid common neighbours of a test are
   // a filter of this list, not a merge of two adjacency lists.  Empty when too large.
   std::vector<int64_t> toff;
-  std::vector<int32_t> tx, tea, teb;
+  std::vector<int32_t> tx, tea, teb, tcnt;  // edge e's triangles: [toff[e], toff[e] + tcnt[e])
   std::vector<int32_t> eu, ev;
 };
 constexpr int64_t kTriListMax = int64_t(1) << 28;  // entries (12 bytes each)
@@ -47,7 +47,7 @@ template <typename Valid>
 int64_t common_e(const Graph& g, int32_t e, Valid valid, int32_t* out, int32_t* ea, int32_t* eb) {
   if (g.toff.empty()) return common(g, g.eu[e], g.ev[e], valid, out, ea, eb);
   int64_t k = 0;
-  for (int64_t t = g.toff[e], te = g.toff[e + 1]; t < te; t++) {
+  for (int64_t t = g.toff[e], te = t + g.tcnt[e]; t < te; t++) {
     const int32_t e1 = g.tea[t], e2 = g.teb[t];
     if (!valid(e1) || !valid(e2)) continue;
     if (ea) {
@@ -119,15 +119,34 @@ namespace ssq {
 
 void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kept, int threads) {
   if (ne <= 0) return;
-  std::vector<int32_t> lab((size_t)(2 * ne));
-  for (int64_t i = 0; i < ne; i++) { lab[2 * i] = u[i]; lab[2 * i + 1] = v[i]; }
-  std::sort(lab.begin(), lab.end());
-  lab.erase(std::unique(lab.begin(), lab.end()), lab.end());
-  int64_t nv = (int64_t)lab.size();
-  std::vector<int32_t> eu((size_t)ne), ev((size_t)ne);
+  // compact vertex labels to 0..nv-1 in increasing label order
+  int32_t lmin = u[0], lmax = u[0];
   for (int64_t i = 0; i < ne; i++) {
-    eu[i] = (int32_t)(std::lower_bound(lab.begin(), lab.end(), u[i]) - lab.begin());
-    ev[i] = (int32_t)(std::lower_bound(lab.begin(), lab.end(), v[i]) - lab.begin());
+    lmin = std::min(lmin, std::min(u[i], v[i]));
+    lmax = std::max(lmax, std::max(u[i], v[i]));
+  }
+  std::vector<int32_t> eu((size_t)ne), ev((size_t)ne);
+  int64_t nv = 0;
+  const int64_t range = (int64_t)lmax - lmin + 1;
+  if (range <= 8 * ne + 1024) {  // labels in a small range: a direct table
+    std::vector<int32_t> id((size_t)range, -1);
+    for (int64_t i = 0; i < ne; i++) id[(size_t)(u[i] - lmin)] = id[(size_t)(v[i] - lmin)] = 0;
+    for (int64_t x = 0; x < range; x++)
+      if (id[(size_t)x] == 0) id[(size_t)x] = (int32_t)nv++;
+    for (int64_t i = 0; i < ne; i++) {
+      eu[i] = id[(size_t)(u[i] - lmin)];
+      ev[i] = id[(size_t)(v[i] - lmin)];
+    }
+  } else {
+    std::vector<int32_t> lab((size_t)(2 * ne));
+    for (int64_t i = 0; i < ne; i++) { lab[2 * i] = u[i]; lab[2 * i + 1] = v[i]; }
+    std::sort(lab.begin(), lab.end());
+    lab.erase(std::unique(lab.begin(), lab.end()), lab.end());
+    nv = (int64_t)lab.size();
+    for (int64_t i = 0; i < ne; i++) {
+      eu[i] = (int32_t)(std::lower_bound(lab.begin(), lab.end(), u[i]) - lab.begin());
+      ev[i] = (int32_t)(std::lower_bound(lab.begin(), lab.end(), v[i]) - lab.begin());
+    }
   }
   Graph g;
   g.off.assign((size_t)nv + 1, 0);
@@ -170,24 +189,21 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
     work();
     for (auto& t : pool) t.join();
   };
-  {  // triangle lists of every edge
-    std::vector<int64_t> cnt((size_t)ne + 1, 0);
-    parallel_edges([&](int64_t e) {
-      cnt[e + 1] = common(g, eu[e], ev[e], [](int32_t) { return true; }, nullptr, nullptr, nullptr);
-    });
-    for (int64_t e = 0; e < ne; e++) cnt[e + 1] += cnt[e];
-    if (cnt[ne] <= kTriListMax) {
-      g.tx.resize((size_t)cnt[ne]);
-      g.tea.resize((size_t)cnt[ne]);
-      g.teb.resize((size_t)cnt[ne]);
+  {  // triangle lists of every edge, in one pass: edge e gets min(deg a, deg b) slots
+    std::vector<int64_t> cap((size_t)ne + 1, 0);
+    for (int64_t e = 0; e < ne; e++)
+      cap[e + 1] = cap[e] + std::min(g.off[eu[e] + 1] - g.off[eu[e]], g.off[ev[e] + 1] - g.off[ev[e]]);
+    if (cap[ne] <= kTriListMax) {
+      g.tx.resize((size_t)cap[ne]);
+      g.tea.resize((size_t)cap[ne]);
+      g.teb.resize((size_t)cap[ne]);
+      g.tcnt.resize((size_t)ne);
       parallel_edges([&](int64_t e) {
-        const int64_t o = cnt[e];
-        int32_t* X = g.tx.data() + o;
-        int32_t* A = g.tea.data() + o;
-        int32_t* B = g.teb.data() + o;
-        common(g, eu[e], ev[e], [](int32_t) { return true; }, X, A, B);
+        const int64_t o = cap[e];
+        g.tcnt[e] = (int32_t)common(g, eu[e], ev[e], [](int32_t) { return true; }, g.tx.data() + o,
+                                    g.tea.data() + o, g.teb.data() + o);
       });
-      g.toff.swap(cnt);
+      g.toff.swap(cap);
     }
   }
 

@@ -402,67 +402,26 @@ __global__ void __launch_bounds__(kCompactBlock) k_key_count(int64_t n, KeyF key
       if (kk < nkeys && mine[kk]) atomicAdd(&s[kk], mine[kk]);
   }
   __syncthreads();
-  if (threadIdx.x < (unsigned)nkeys) counts[(int64_t)blockIdx.x * nkeys + threadIdx.x] = s[threadIdx.x];
-}
-
-__device__ __forceinline__ int64_t wave_incl_scan64(int64_t v) {
-  const int lane = lane_id();
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int64_t o = __shfl_up(v, d, kWave);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
-// Exclusive scan over workgroups, keys laid out key-major in the output: the base of key k
-// is the total of all smaller keys.  One workgroup of kKeyScanThreads; totals[k] receives
-// each key's count.
-constexpr int kKeyScanThreads = 1024;
-__global__ void __launch_bounds__(kKeyScanThreads) k_key_scan(int64_t nblocks, uint32_t* counts, int nkeys,
-                                                             int64_t* totals) {
-  __shared__ int64_t s_w[kKeyScanThreads / kWave];
-  __shared__ int64_t s_tot[kMaxDims];
-  const int w = threadIdx.x / kWave, lane = lane_id();
-  const int64_t per = (nblocks + kKeyScanThreads - 1) / kKeyScanThreads;
-  const int64_t b0 = min((int64_t)threadIdx.x * per, nblocks);
-  const int64_t b1 = min(b0 + per, nblocks);
-  int64_t key_base = 0;
-  for (int k = 0; k < nkeys; k++) {
-    int64_t sum = 0;
-    for (int64_t b = b0; b < b1; b++) sum += counts[b * nkeys + k];
-    const int64_t incl = wave_incl_scan64(sum);
-    if (lane == kWave - 1) s_w[w] = incl;
-    __syncthreads();
-    int64_t before = 0, tot = 0;
-    for (int ww = 0; ww < kKeyScanThreads / kWave; ww++) {
-      const int64_t v = s_w[ww];
-      if (ww < w) before += v;
-      tot += v;
-    }
-    int64_t run = key_base + before + incl - sum;
-    for (int64_t b = b0; b < b1; b++) {
-      const uint32_t v = counts[b * nkeys + k];
-      counts[b * nkeys + k] = (uint32_t)run;  // base offsets fit: n < 2^31
-      run += v;
-    }
-    if (threadIdx.x == 0) s_tot[k] = tot;
-    key_base += tot;
-    __syncthreads();  // s_w is reused by the next key
-  }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < nkeys; k++) totals[k] = s_tot[k];
+  // key-major: an exclusive scan over counts[k * nb + b] is the stable partition's base of
+  // (key k, workgroup b)
+  if (threadIdx.x < (unsigned)nkeys) counts[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s[threadIdx.x];
 }
 
 // Stable scatter: cells keep their order within each key (sub-tiles, then waves, then lanes).
+// per-key totals from the scanned key-major counts (bases[nkeys * nb] is the grand total)
+__global__ void k_key_totals(int nkeys, int64_t nb, const int64_t* bases, int64_t* totals) {
+  if (threadIdx.x < (unsigned)nkeys) totals[threadIdx.x] = bases[(threadIdx.x + 1) * nb] - bases[threadIdx.x * nb];
+}
+
 template <typename KeyF, typename EmitF>
-__global__ void __launch_bounds__(kCompactBlock) k_key_scatter(int64_t n, KeyF key, const uint32_t* bases, int nkeys,
+__global__ void __launch_bounds__(kCompactBlock) k_key_scatter(int64_t n, KeyF key, const int64_t* bases, int nkeys,
                                                               EmitF emit) {
   __shared__ uint32_t s_wave[kCompactBlock / kWave][kMaxDims];
   __shared__ uint32_t s_run[kMaxDims];
   const int lane = lane_id();
   const int w = threadIdx.x / kWave;
   const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  if (threadIdx.x < (unsigned)nkeys) s_run[threadIdx.x] = bases[(int64_t)blockIdx.x * nkeys + threadIdx.x];
+  if (threadIdx.x < (unsigned)nkeys) s_run[threadIdx.x] = (uint32_t)bases[(int64_t)threadIdx.x * gridDim.x + blockIdx.x];
   for (int j = 0; j < kCompactSub; j++) {
     const int64_t i = (int64_t)blockIdx.x * kCompactItems + j * kCompactBlock + threadIdx.x;
     const int k = i < n ? key(i) : -1;
@@ -1896,7 +1855,7 @@ class Engine {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
                       &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
                       &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_,
-                      &b_wcnt_, &b_woff_, &b_scrc_, &b_scrr_})
+                      &b_wcnt_, &b_woff_, &b_scrc_, &b_scrr_, &b_kbase_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
     if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
@@ -1955,7 +1914,7 @@ class Engine {
   std::string err_;
   DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
       b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
-      b_drem_, b_scan_, b_pmeta_, b_prem_, b_wcnt_, b_woff_, b_scrc_, b_scrr_;
+      b_drem_, b_scan_, b_pmeta_, b_prem_, b_wcnt_, b_woff_, b_scrc_, b_scrr_, b_kbase_;
   int64_t arena_hint_ = 0;
   bool no_block_dual_ = false;  // test hook: always use the atomic anti-transpose
   bool timing_ = false;
@@ -1970,8 +1929,10 @@ template <typename KeyF, typename EmitF>
 int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host, int slot) {
   const int64_t nb = std::max<int64_t>(1, (n + kCompactItems - 1) / kCompactItems);
   uint32_t* counts;
-  int rc = ensure(b_cnt_, (size_t)(nb * nkeys + 16), &counts);
-  if (rc) return rc;
+  int64_t* bases;
+  int rc;
+  if ((rc = ensure(b_cnt_, (size_t)(nb * nkeys + 16), &counts)) || (rc = ensure(b_kbase_, (size_t)(nb * nkeys + 1), &bases)))
+    return rc;
   int64_t* totals = tot_ + slot;
   if (n > 0) {
     hipLaunchKernelGGL(k_key_count<KeyF>, dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key, counts,
@@ -1979,10 +1940,11 @@ int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_
   } else {
     HIPCHK(hipMemsetAsync(counts, 0, sizeof(uint32_t) * nkeys, stream_));
   }
-  hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(kKeyScanThreads), 0, stream_, nb, counts, nkeys, totals);
+  if ((rc = scan_counts((const int32_t*)counts, nb * nkeys, bases))) return rc;
+  hipLaunchKernelGGL(k_key_totals, dim3(1), dim3(64), 0, stream_, nkeys, nb, (const int64_t*)bases, totals);
   if (n > 0)
     hipLaunchKernelGGL((k_key_scatter<KeyF, EmitF>), dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key,
-                       counts, nkeys, emit);
+                       (const int64_t*)bases, nkeys, emit);
   HIPCHK(hipGetLastError());
   if (!totals_host) return SSQ_OK;  // the caller reads tot_[slot..] at its next sync
   HIPCHK(hipMemcpyAsync(host_tot_ + slot, totals, sizeof(int64_t) * nkeys, hipMemcpyDeviceToHost, stream_));

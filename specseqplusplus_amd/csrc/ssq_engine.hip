@@ -863,40 +863,53 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
         __syncthreads();
         r_len = sh_count;
         // new low and the row below it: the two highest set bits, scanning down from the old
-        // low's word; the second is looked for at most one range past the first
-        int32_t t1 = -1, t2 = -1, ranges_past = 0;
-        int32_t base = r_len > 0 ? (l - lo) >> 5 : -1;  // r_len is uniform (read from LDS)
-        for (; base >= 0; base -= kLongThreads) {  // t1, t2 are uniform: the exits are too
-          const int32_t w = base - tid;
-          int32_t x1 = -1, x2 = -1;
-          if (w >= 0) {
-            uint32_t m = bits[w];
-            if (m) {
-              const int h = 31 - __clz(m);
-              x1 = lo + (w << 5) + h;
-              m &= ~(1u << h);
-              if (m) x2 = lo + (w << 5) + (31 - __clz(m));
+        // low's word.  Wavefront 0 alone scans, 64 words per step, and publishes them; the
+        // second is looked for up to 8 steps past the first.  Two barriers per addition:
+        // every read of this step's LDS values happens before the second, every write of the
+        // next step after it.
+        if (tid < kWave) {
+          const int lane = tid;
+          int32_t t1 = -1, t2 = -1, past = 0;
+          int32_t base = r_len > 0 ? (l - lo) >> 5 : -1;  // r_len is uniform (read from LDS)
+          for (; base >= 0; base -= kWave) {  // t1, t2 are uniform in the wavefront: so are the exits
+            const int32_t w = base - lane;
+            int32_t x1 = -1, x2 = -1;
+            if (w >= 0) {
+              uint32_t m = bits[w];
+              if (m) {
+                const int h = 31 - __clz(m);
+                x1 = lo + (w << 5) + h;
+                m &= ~(1u << h);
+                if (m) x2 = lo + (w << 5) + (31 - __clz(m));
+              }
             }
+            for (int k = kWave / 2; k > 0; k >>= 1) {
+              const int32_t o1 = __shfl_xor(x1, k, kWave), o2 = __shfl_xor(x2, k, kWave);
+              top2_merge(x1, x2, o1, o2);
+            }
+            if (t1 < 0) {
+              t1 = x1;
+              t2 = x2;
+            } else {
+              t2 = x1;
+            }
+            if (t2 >= 0 || (t1 >= 0 && r_len == 1)) break;
+            if (t1 >= 0 && ++past >= 8) break;
           }
-          block_top2(x1, x2, red);
-          if (t1 < 0) {
-            t1 = x1;
-            t2 = x2;
-          } else {
-            t2 = x1;
+          if (lane == 0) {
+            red[0] = t1;
+            red[1] = t2 >= 0 ? t2 : ((base < 0 || r_len <= 1) ? -1 : kUnknown);
           }
-          if (t2 >= 0 || (t1 >= 0 && r_len == 1)) break;
-          if (t1 >= 0 && ++ranges_past >= 2) break;
         }
-        const int32_t newl = t1;
-        a2 = t2 >= 0 ? t2 : ((base < 0 || r_len <= 1) ? -1 : kUnknown);
+        __syncthreads();
+        const int32_t newl = red[0];
+        a2 = red[1];
         n_dense += 1;
         n_dbytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
         max_len = max(max_len, (uint32_t)(a_len + b_len));
         chain += 1;
         a_len = r_len;
         l = r_len ? newl : -1;
-        __syncthreads();
       } else {
         // tiled merge by wave 0 into a fresh slot
         if (tid == 0) {

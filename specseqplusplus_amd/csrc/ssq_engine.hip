@@ -1235,11 +1235,8 @@ __global__ void __launch_bounds__(kWave * kPlaceWaves) k_win_place(int64_t n, in
                                                                    const uint8_t* scr_r, int64_t* dcolptr,
                                                                    int32_t* drows, uint32_t* overflow) {
   __shared__ int32_t vb[kPlaceWaves][kWinCap];
-  __shared__ uint32_t cnt[kPlaceWaves][kWinRows], off[kPlaceWaves][kWinRows];
   const int wv = threadIdx.x / kWave, lane = lane_id();
   int32_t* v = vb[wv];
-  uint32_t* cn = cnt[wv];
-  uint32_t* of = off[wv];
   const int64_t tot = woff[nw];  // entries of the dual (columns marked removed contribute none)
   if (blockIdx.x == 0 && threadIdx.x == 0) dcolptr[n] = tot;
   for (int64_t win = (int64_t)blockIdx.x * kPlaceWaves + wv; win < nw; win += (int64_t)gridDim.x * kPlaceWaves) {
@@ -1251,30 +1248,47 @@ __global__ void __launch_bounds__(kWave * kPlaceWaves) k_win_place(int64_t n, in
       if (lane == 0) *overflow = 1u;
       continue;
     }
-    if (lane < kWinRows) cn[lane] = 0;
-    wave_sync();
-    for (int32_t q = lane; q < T; q += kWave) atomicAdd(&cn[scr_r[b + q]], 1u);
-    wave_sync();
-    if (lane == 0) {  // rows in decreasing order: the dual lists them by increasing n-1-r
-      uint32_t run = 0;
-      for (int32_t i = R - 1; i >= 0; i--) {
-        of[i] = run;
-        run += cn[i];
+    // rows' counts by ballots: lane i (< kWinRows) holds row i's count
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    uint32_t mycnt = 0;
+    for (int32_t q0 = 0; q0 < T; q0 += kWave) {
+      const int32_t rr = q0 + lane < T ? (int32_t)scr_r[b + q0 + lane] : -1;
+#pragma unroll
+      for (int i = 0; i < kWinRows; i++) {
+        const uint32_t c = (uint32_t)__popcll(__ballot(rr == i));
+        if (lane == i) mycnt += c;
       }
     }
-    wave_sync();
+    // rows in decreasing order (the dual lists them by increasing n-1-r): row i starts after
+    // all rows above it, i.e. at T - (rows 0..i)
+    uint32_t incl = lane < R ? mycnt : 0u;
+    for (int k = 1; k < kWave; k <<= 1) {
+      const uint32_t y = __shfl_up(incl, k, kWave);
+      if (lane >= k) incl += y;
+    }
+    uint32_t cur = (uint32_t)T - incl;  // lane i < R: first position of row i
     const int64_t g0 = tot - e;
-    if (lane < R) dcolptr[n - 1 - (r0 + lane)] = g0 + of[lane];
-    wave_sync();
-    for (int32_t q = lane; q < T; q += kWave) {
-      const uint32_t pos = atomicAdd(&of[scr_r[b + q]], 1u);
-      v[pos] = (int32_t)(n - 1 - scr_c[b + q]);
+    if (lane < R) dcolptr[n - 1 - (r0 + lane)] = g0 + cur;
+    const int32_t lo = lane < R ? (int32_t)cur : 0;
+    const int32_t len = lane < R ? (int32_t)mycnt : 0;
+    // placement: rank within the row from the ballot, the row's cursor from its lane
+    for (int32_t q0 = 0; q0 < T; q0 += kWave) {
+      const bool in = q0 + lane < T;
+      const int32_t rr = in ? (int32_t)scr_r[b + q0 + lane] : -1;
+      const int32_t cc = in ? scr_c[b + q0 + lane] : 0;
+      uint32_t pos = 0;
+#pragma unroll
+      for (int i = 0; i < kWinRows; i++) {
+        const uint64_t m = __ballot(rr == i);
+        const uint32_t base = (uint32_t)__shfl((int)cur, i, kWave);
+        if (rr == i) pos = base + (uint32_t)__popcll(m & below);
+        if (lane == i) cur += (uint32_t)__popcll(m);
+      }
+      if (in) v[pos] = (int32_t)(n - 1 - cc);
     }
     wave_sync();
     // each dual column ascending: short ones by their lane in registers, the others by the
     // whole wavefront
-    const int32_t len = lane < R ? (int32_t)cn[lane] : 0;
-    const int32_t lo = lane < R ? (int32_t)of[lane] - len : 0;
     if (len > 1 && len <= 16) {
       int32_t x[16];
 #pragma unroll

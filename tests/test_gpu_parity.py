@@ -284,9 +284,9 @@ def test_batch_reduce_points_equals_alone(h, maxdim):
 
 
 @pytest.mark.parametrize("general", [False, True])
-def test_dual_block_transpose_matches_general(monkeypatch, general):
-    """Cohomology through the per-block anti-transpose (independent complexes found by their
-    cut points) and through the general atomic one give the oracle's pairs."""
+def test_dual_window_transpose_matches_general(monkeypatch, general):
+    """Cohomology through the row-window anti-transpose and through the atomic one (forced by
+    the test hook) give the oracle's pairs, batched and device-resident."""
     from scipy.spatial.distance import pdist
     from specseqplusplus_amd import configs as CF
     if general:
@@ -312,5 +312,41 @@ def test_dual_block_transpose_matches_general(monkeypatch, general):
         npairs, _ = hd.reduce_resident(dm, out, p)
         allp = out.numpy(2 * npairs).reshape(-1, 2)
         assert np.array_equal(allp, O.twist_pairs(O.Matrix(bd.colptr, bd.rows, bd.dims)))
+    finally:
+        hd.close()
+
+
+def test_dual_window_overflow_falls_back():
+    """A vertex on 3000 edges puts 3000 entries into one 16-row window of the anti-transpose
+    (more than a window holds in LDS): the engine must fall back to the atomic path and stay
+    exact, alone and next to ordinary complexes in a batch."""
+    k = 3000
+    nv = k + 1
+    colptr = [0] * (nv + 1)
+    rows = []
+    for i in range(1, k + 1):  # edges (0, i), then the triangles (0, i, i+1) for the first few i
+        rows += [0, i]
+        colptr.append(len(rows))
+    ntri = 40
+    for i in range(1, ntri + 1):
+        # edge ids: edge (0, i) is column nv + i - 1; add edge (i, i+1) first
+        rows += [i, i + 1]
+        colptr.append(len(rows))
+    base_e2 = nv + k
+    for i in range(1, ntri + 1):
+        rows += sorted([nv + i - 1, nv + i, base_e2 + i - 1])
+        colptr.append(len(rows))
+    dims = [0] * nv + [1] * k + [1] * ntri + [2] * ntri
+    m = S.Matrix(np.array(colptr, np.int64), np.array(rows, np.int32), np.array(dims, np.int32))
+    mo = O.Matrix(m.colptr, m.rows, m.dims)
+    ref = O.twist_pairs(mo)
+    hd = S.Handle(0)
+    try:
+        p = S.params(dualize=1, ss_block_size=1 << 30)
+        assert np.array_equal(hd.reduce(m, p)[0], ref)
+        small = as_product(c1_matrix())
+        outs = hd.batch_reduce([small, m, small], p)
+        assert np.array_equal(outs[1][0], ref)
+        assert np.array_equal(outs[0][0], outs[2][0])
     finally:
         hd.close()

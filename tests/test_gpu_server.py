@@ -33,3 +33,32 @@ def test_server_results_equal_alone():
             pairs, ess, lat = srv.poll(t)
             assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))
             assert lat > 0
+
+
+def test_server_mixed_dimensions_and_parameters():
+    """Point clouds of two ambient dimensions and of simplex dimension 2 (GPU-built) and 3
+    (host-built), interleaved with matrices, under a small cell cap and small batches: every
+    batch holds one kind and one shape, and every result equals the oracle's pairs."""
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(30):
+        kind = i % 3
+        if kind == 0:
+            X = rng.random((20 + i, 3)).astype(np.float32)
+            reqs.append(("pts", X, 0.35, 2))
+        elif kind == 1:
+            X = rng.random((12 + i // 2, 4)).astype(np.float32)
+            reqs.append(("pts", X, 0.7, 3 if i % 2 else 2))
+        else:
+            X = rng.random((25 + i, 3)).astype(np.float32)
+            reqs.append(("mat", S.vr_build(X, 0.3, 2), None, None))
+    with Server(streams=2, max_batch=5, max_batch_cells=20000, p=S.params(),
+                points_p=S.params(dualize=1, ss_block_size=1 << 30)) as srv:
+        tickets = []
+        for kind, a, r, md in reqs:
+            tickets.append(srv.submit_points(a, r, md) if kind == "pts" else srv.submit_matrix(a))
+        srv.wait_all()
+        for (kind, a, r, md), t in zip(reqs, tickets):
+            m = S.vr_build(a, r, md) if kind == "pts" else a
+            pairs, ess, lat = srv.poll(t)
+            assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims))), (kind, md)

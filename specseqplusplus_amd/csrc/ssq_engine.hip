@@ -5,11 +5,17 @@
 //                collapser, coface propagation on device            (specseq.h:410-438,
 //                                                                     kernel.h:190-198)
 //   model bit 0  high-dimension guided clearing of max-dim cells      (kernel.h:121-292)
+//   dualize      the anti-transpose by 16-row windows (cohomology);
+//                pairs map back as (b', d') -> (n-1-d', n-1-b')       (specseq.h:303-363,
+//                                                                     kernel.h:521-580)
 //   init passes  low / leftmost-one, apparent pairs, clearing, compression
 //                                                                     (kernel.h:444-520)
 //   page loop    per dimension from high to low, spectral-sequence pages of columns x
 //                rows; each page runs claim/add rounds until every active column is a
 //                pivot, zero or below the page's row floor            (specseq.h:592-704)
+//                A round: claims, the claimants' pivot metadata, additions by one
+//                wavefront per column (LDS merge), then additions of the columns that
+//                outgrew a wavefront's LDS by one workgroup each (LDS bit set).
 //   twist        rows of the dimension's pivots are zeroed before the next dimension
 //                                                                     (specseq.h:1039-1046)
 //   pairs        (row, pivot column) in row order                     (specseq.h:1182-1193)

@@ -27,6 +27,7 @@
 // addition adds a column to the left (p < c), which keeps all lower-left ranks and hence the
 // pairs.  A chasing column keeps adding piv[low] while that is a smaller column.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -47,7 +48,10 @@ namespace ssq {
 // device state
 // ------------------------------------------------------------------------------------
 constexpr int kAddWaves = 4;           // wavefronts per add workgroup
-constexpr int kTile = 1024;            // LDS entries per staging buffer
+#ifndef SSQ_KTILE
+#define SSQ_KTILE 1024
+#endif
+constexpr int kTile = SSQ_KTILE;       // LDS entries per staging buffer
 constexpr int kTileBuf = kTile + kWave;
 constexpr int kMaxDims = 8;            // cell dimensions 0..7
 
@@ -363,12 +367,19 @@ __global__ void k_pivmeta(Dev d, int r) {
 // one atomic per wavefront (a plain per-thread atomic on one address serialises millions)
 __global__ void k_maxdim(int64_t n, const int32_t* dims, int32_t* out, uint32_t* bad) {
   int32_t mx = -1, b = 0;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = base + threadIdx.x;
-    if (c < n) {
-      const int32_t x = dims[c];
-      if (x < 0 || x >= kMaxDims) b = 1;
-      else mx = max(mx, x);
+  // four independent loads per thread and trip: enough bytes in flight for HBM
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += 4 * stride) {
+    int32_t x[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t c = base + j * stride + threadIdx.x;
+      x[j] = c < n ? dims[c] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (x[j] < 0 || x[j] >= kMaxDims) b = 1;
+      else mx = max(mx, x[j]);
     }
   }
   mx = wave_max(mx);
@@ -1057,275 +1068,66 @@ __global__ void k_dual_dims(int64_t n, const int32_t* dims, int32_t maxd, const 
   }
 }
 
+// ---- anti-transpose by a stable radix sort ---------------------------------------------
+// Entry k = (row r, column c) becomes (key n-1-r, value n-1-c), written at position nnz-1-k:
+// a stable sort by key then lists each dual column's rows in ascending order (larger c
+// first).  Columns marked removed get key n and sort past every dual column.
+__global__ void k_dual_keys(int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows,
+                            const uint8_t* removed, uint32_t* keys, int32_t* vals) {
+  GRID_STRIDE(c, n) {
+    const bool gone = removed && removed[c];
+    const int32_t v = (int32_t)(n - 1 - c);
+    for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
+      const int64_t pos = nnz - 1 - k;
+      keys[pos] = gone ? (uint32_t)n : (uint32_t)(n - 1 - rows[k]);
+      vals[pos] = v;
+    }
+  }
+}
+
+// Runs of equal keys in the sorted keys: first[k] = start of key k's run (only where key k
+// occurs); then len[k] = run length (len[] zeroed beforehand, so absent keys stay 0).  Each
+// thread takes four consecutive keys in one 16-byte load (the key buffer is 16-byte aligned);
+// 0xffffffff stands for "no neighbour" (keys are at most n < 2^31).
+__device__ __forceinline__ void load_keys4(const uint32_t* keys, int64_t nnz, int64_t i0, uint32_t (&k)[4]) {
+  if (i0 + 3 < nnz) {
+    const uint4 v = *(const uint4*)(keys + i0);
+    k[0] = v.x, k[1] = v.y, k[2] = v.z, k[3] = v.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) k[j] = i0 + j < nnz ? keys[i0 + j] : 0xffffffffu;
+  }
+}
+
+__global__ void k_run_first(int64_t n, int64_t nnz, const uint32_t* keys, int64_t* first) {
+  GRID_STRIDE(q, (nnz + 3) / 4) {
+    const int64_t i0 = 4 * q;
+    uint32_t k[4];
+    load_keys4(keys, nnz, i0, k);
+    uint32_t prev = i0 > 0 ? keys[i0 - 1] : 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k[j] < (uint64_t)n && k[j] != prev) first[k[j]] = i0 + j;
+      prev = k[j];
+    }
+  }
+}
+
+__global__ void k_run_len(int64_t n, int64_t nnz, const uint32_t* keys, const int64_t* first, int32_t* len) {
+  GRID_STRIDE(q, (nnz + 3) / 4) {
+    const int64_t i0 = 4 * q;
+    uint32_t k[4];
+    load_keys4(keys, nnz, i0, k);
+    const uint32_t next = i0 + 4 < nnz ? keys[i0 + 4] : 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t nx = j < 3 ? k[j + 1] : next;
+      if (k[j] < (uint64_t)n && k[j] != nx) len[k[j]] = (int32_t)(i0 + j + 1 - first[k[j]]);
+    }
+  }
+}
+
 __device__ void heapsort_i32(int32_t* a, int64_t n);
-
-// ---- anti-transpose by row windows ----------------------------------------------------
-// Dual column n-1-r lists n-1-c for every column c holding row r.  Rows are cut into windows
-// of kWinRows; (1) workgroups count the entries of each window over their chunk of columns
-// in an LDS histogram and add the totals with one global atomic per (chunk, window), (2) a
-// scan gives each window its run, (3) the same chunks place their entries into their
-// windows' runs in scratch (one reservation per chunk and window, ranks in LDS), (4) one
-// wavefront per window counts its rows, places and sorts each dual column in LDS and writes
-// the window's run out contiguously.  Block-diagonal batches keep a chunk's rows within a
-// few complexes, so the histograms absorb almost every entry.
-constexpr int kWinRows = 16;        // rows per window
-constexpr int kWinCap = 2048;       // entries per window (LDS per wavefront)
-constexpr int kHist = 8192;         // windows tracked in LDS by one bucketing workgroup
-constexpr int kChunkCols = 2048;    // columns per bucketing workgroup
-constexpr int kPlaceWaves = 4;
-
-__device__ __forceinline__ int32_t block_min_i32_256(int32_t v, int32_t* red) {
-  for (int k = kWave / 2; k > 0; k >>= 1) v = min(v, __shfl_xor(v, k, kWave));
-  __syncthreads();
-  if (lane_id() == 0) red[threadIdx.x / kWave] = v;
-  __syncthreads();
-  int32_t m = red[0];
-  for (int k = 1; k < (int)(blockDim.x / kWave); k++) m = min(m, red[k]);
-  return m;
-}
-
-__global__ void __launch_bounds__(256) k_win_count(int64_t n, const int64_t* colptr, const int32_t* rows,
-                                                   const uint8_t* removed, uint32_t* wcnt) {
-  __shared__ uint32_t H[kHist];
-  __shared__ int32_t red[4];
-  for (int64_t c0 = (int64_t)blockIdx.x * kChunkCols; c0 < n; c0 += (int64_t)gridDim.x * kChunkCols) {
-    const int64_t c1 = min(n, c0 + kChunkCols);
-    const int64_t k0 = colptr[c0], k1 = colptr[c1];
-    int32_t mn = 0x7fffffff;
-    if (removed) {
-      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
-        if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
-    } else {  // no column is removed: the chunk's entries are one contiguous run
-      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) mn = min(mn, rows[k]);
-    }
-    const int32_t lo = block_min_i32_256(mn, red) / kWinRows;  // windows below lo get nothing
-    for (int i = threadIdx.x; i < kHist; i += blockDim.x) H[i] = 0;
-    __syncthreads();
-    if (removed) {
-      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-        if (removed[c]) continue;
-        for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
-          const int32_t w = rows[k] / kWinRows, d = w - lo;
-          if (d < kHist) atomicAdd(&H[d], 1u);
-          else atomicAdd(&wcnt[w], 1u);
-        }
-      }
-    } else {
-      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        const int32_t w = rows[k] / kWinRows, d = w - lo;
-        if (d < kHist) atomicAdd(&H[d], 1u);
-        else atomicAdd(&wcnt[w], 1u);
-      }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kHist; i += blockDim.x)
-      if (H[i]) atomicAdd(&wcnt[lo + i], H[i]);
-    __syncthreads();
-  }
-}
-
-__global__ void __launch_bounds__(256) k_win_bucket(int64_t n, const int64_t* colptr, const int32_t* rows,
-                                                    const uint8_t* removed, const int64_t* woff, uint32_t* wfill,
-                                                    int32_t* scr_c, uint8_t* scr_r) {
-  __shared__ uint32_t H[kHist];
-  __shared__ uint32_t B[kHist];
-  __shared__ int32_t red[4];
-  for (int64_t c0 = (int64_t)blockIdx.x * kChunkCols; c0 < n; c0 += (int64_t)gridDim.x * kChunkCols) {
-    const int64_t c1 = min(n, c0 + kChunkCols);
-    const int64_t k0 = colptr[c0], k1 = colptr[c1];
-    int32_t mn = 0x7fffffff;
-    if (removed) {
-      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
-        if (!removed[c] && colptr[c + 1] > colptr[c]) mn = min(mn, rows[colptr[c]]);
-    } else {
-      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) mn = min(mn, rows[k]);
-    }
-    const int32_t lo = block_min_i32_256(mn, red) / kWinRows;
-    for (int i = threadIdx.x; i < kHist; i += blockDim.x) H[i] = 0;
-    __syncthreads();
-    if (removed) {
-      for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-        if (removed[c]) continue;
-        for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
-          const int32_t d = rows[k] / kWinRows - lo;
-          if (d < kHist) atomicAdd(&H[d], 1u);
-        }
-      }
-    } else {
-      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        const int32_t d = rows[k] / kWinRows - lo;
-        if (d < kHist) atomicAdd(&H[d], 1u);
-      }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kHist; i += blockDim.x) {  // reserve this chunk's share of each window
-      const uint32_t h = H[i];
-      B[i] = h ? atomicAdd(&wfill[lo + i], h) : 0u;
-      H[i] = 0;
-    }
-    __syncthreads();
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-      if (removed && removed[c]) continue;
-      for (int64_t k = colptr[c], ke = colptr[c + 1]; k < ke; k++) {
-        const int32_t r = rows[k], w = r / kWinRows, d = w - lo;
-        const uint32_t pos = d < kHist ? B[d] + atomicAdd(&H[d], 1u) : atomicAdd(&wfill[w], 1u);
-        const int64_t idx = woff[w] + pos;
-        scr_c[idx] = (int32_t)c;
-        scr_r[idx] = (uint8_t)(r - w * kWinRows);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Bitonic sort of N values in registers (fully unrolled: static indices).
-template <int N>
-__device__ __forceinline__ void reg_bitonic(int32_t (&x)[N]) {
-#pragma unroll
-  for (int k = 2; k <= N; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1)
-#pragma unroll
-      for (int i = 0; i < N; i++) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool up = (i & k) == 0;
-          const int32_t a = x[i], b = x[l];
-          const bool sw = up ? (a > b) : (a < b);
-          x[i] = sw ? b : a;
-          x[l] = sw ? a : b;
-        }
-      }
-}
-
-// Ascending sort of v[0:len) (len <= 256, in LDS) by one wavefront: 4 values per lane,
-// element i = 4*lane + t, bitonic across registers and lanes.
-__device__ __forceinline__ void wave_sort256(int32_t* v, int32_t len, int lane) {
-  int32_t x[4];
-#pragma unroll
-  for (int t = 0; t < 4; t++) x[t] = 4 * lane + t < len ? v[4 * lane + t] : 0x7fffffff;
-#pragma unroll
-  for (int k = 2; k <= 256; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j >= 4) {
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-          const int i = 4 * lane + t;
-          const int32_t y = __shfl_xor(x[t], j >> 2, kWave);
-          const bool up = (i & k) == 0, lower = (i & j) == 0;
-          x[t] = (lower == up) ? min(x[t], y) : max(x[t], y);
-        }
-      } else {
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-          const int l = t ^ j;
-          if (l > t) {
-            const int i = 4 * lane + t;
-            const bool up = (i & k) == 0;
-            const int32_t a = x[t], b = x[l];
-            const bool sw = up ? (a > b) : (a < b);
-            x[t] = sw ? b : a;
-            x[l] = sw ? a : b;
-          }
-        }
-      }
-    }
-#pragma unroll
-  for (int t = 0; t < 4; t++)
-    if (4 * lane + t < len) v[4 * lane + t] = x[t];
-}
-
-__global__ void __launch_bounds__(kWave * kPlaceWaves) k_win_place(int64_t n, int64_t nw,
-                                                                   const int64_t* woff, const int32_t* scr_c,
-                                                                   const uint8_t* scr_r, int64_t* dcolptr,
-                                                                   int32_t* drows, uint32_t* overflow) {
-  __shared__ int32_t vb[kPlaceWaves][kWinCap];
-  const int wv = threadIdx.x / kWave, lane = lane_id();
-  int32_t* v = vb[wv];
-  const int64_t tot = woff[nw];  // entries of the dual (columns marked removed contribute none)
-  if (blockIdx.x == 0 && threadIdx.x == 0) dcolptr[n] = tot;
-  for (int64_t win = (int64_t)blockIdx.x * kPlaceWaves + wv; win < nw; win += (int64_t)gridDim.x * kPlaceWaves) {
-    const int64_t b = woff[win], e = woff[win + 1];
-    const int32_t T = (int32_t)(e - b);
-    const int64_t r0 = win * kWinRows;
-    const int32_t R = (int32_t)min((int64_t)kWinRows, n - r0);
-    if (T > kWinCap) {  // the caller falls back to the general anti-transpose
-      if (lane == 0) *overflow = 1u;
-      continue;
-    }
-    // rows' counts by ballots: lane i (< kWinRows) holds row i's count
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
-    uint32_t mycnt = 0;
-    for (int32_t q0 = 0; q0 < T; q0 += kWave) {
-      const int32_t rr = q0 + lane < T ? (int32_t)scr_r[b + q0 + lane] : -1;
-#pragma unroll
-      for (int i = 0; i < kWinRows; i++) {
-        const uint32_t c = (uint32_t)__popcll(__ballot(rr == i));
-        if (lane == i) mycnt += c;
-      }
-    }
-    // rows in decreasing order (the dual lists them by increasing n-1-r): row i starts after
-    // all rows above it, i.e. at T - (rows 0..i)
-    uint32_t incl = lane < R ? mycnt : 0u;
-    for (int k = 1; k < kWave; k <<= 1) {
-      const uint32_t y = __shfl_up(incl, k, kWave);
-      if (lane >= k) incl += y;
-    }
-    uint32_t cur = (uint32_t)T - incl;  // lane i < R: first position of row i
-    const int64_t g0 = tot - e;
-    if (lane < R) dcolptr[n - 1 - (r0 + lane)] = g0 + cur;
-    const int32_t lo = lane < R ? (int32_t)cur : 0;
-    const int32_t len = lane < R ? (int32_t)mycnt : 0;
-    // placement: rank within the row from the ballot, the row's cursor from its lane
-    for (int32_t q0 = 0; q0 < T; q0 += kWave) {
-      const bool in = q0 + lane < T;
-      const int32_t rr = in ? (int32_t)scr_r[b + q0 + lane] : -1;
-      const int32_t cc = in ? scr_c[b + q0 + lane] : 0;
-      uint32_t pos = 0;
-#pragma unroll
-      for (int i = 0; i < kWinRows; i++) {
-        const uint64_t m = __ballot(rr == i);
-        const uint32_t base = (uint32_t)__shfl((int)cur, i, kWave);
-        if (rr == i) pos = base + (uint32_t)__popcll(m & below);
-        if (lane == i) cur += (uint32_t)__popcll(m);
-      }
-      if (in) v[pos] = (int32_t)(n - 1 - cc);
-    }
-    wave_sync();
-    // each dual column ascending: short ones by their lane in registers, the others by the
-    // whole wavefront
-    if (len > 1 && len <= 16) {
-      int32_t x[16];
-#pragma unroll
-      for (int i = 0; i < 16; i++) x[i] = i < len ? v[lo + i] : 0x7fffffff;
-      reg_bitonic<16>(x);
-#pragma unroll
-      for (int i = 0; i < 16; i++)
-        if (i < len) v[lo + i] = x[i];
-    } else if (len > 16 && len <= 32) {
-      int32_t x[32];
-#pragma unroll
-      for (int i = 0; i < 32; i++) x[i] = i < len ? v[lo + i] : 0x7fffffff;
-      reg_bitonic<32>(x);
-#pragma unroll
-      for (int i = 0; i < 32; i++)
-        if (i < len) v[lo + i] = x[i];
-    }
-    wave_sync();
-    uint64_t longs = __ballot(len > 32);
-    while (longs) {
-      const int src = __ffsll((unsigned long long)longs) - 1;
-      longs &= longs - 1;
-      const int32_t L = __shfl(len, src, kWave), B0 = __shfl(lo, src, kWave);
-      if (L <= 256) wave_sort256(v + B0, L, lane);
-      else if (lane == 0) heapsort_i32(v + B0, L);
-      wave_sync();
-    }
-    for (int32_t q = lane; q < T; q += kWave) drows[g0 + q] = v[q];
-    wave_sync();
-  }
-}
 
 // Sort every segment v[off[s] : off[s+1]) ascending.  Segments longer than a lane's share
 // are sorted by a wavefront: bitonic in registers up to 64 entries, in LDS up to kSortLds,
@@ -1426,42 +1228,63 @@ __global__ void __launch_bounds__(256) k_sort_segments(int64_t nseg, const int64
 }
 
 // exclusive scan of int32 counts into int64 offsets out[0..n] (out[n] = total)
-constexpr int kScanBlock = 1024;
-__global__ void k_scan_partial(const int32_t* in, int64_t n, int64_t* bsum) {
-  __shared__ int64_t s[256];
-  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
-  int64_t acc = 0;
-  for (int t = threadIdx.x; t < kScanBlock; t += 256)
-    if (b0 + t < n) acc += in[b0 + t];
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
-    __syncthreads();
+// Exclusive scan of int32 counts into int64 offsets (out[n] = total) in three passes: tile
+// sums, a scan of the tile sums, then each tile scanned with its base.  A tile is kScanTile
+// counts: 16 per thread, loaded coalesced and staged through LDS (one pad slot per 16
+// entries keeps a thread's 16 consecutive entries off a single bank).
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 1; k < kWave; k <<= 1) {
+    const int64_t y = __shfl_up(v, k, kWave);
+    if (lane >= k) v += y;
   }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+  return v;
 }
 
-__global__ void k_scan_bsums(int64_t nb, int64_t* bsum) {
-  __shared__ int64_t part[1024];
-  const int64_t per = (nb + blockDim.x - 1) / blockDim.x;
+__global__ void __launch_bounds__(kScanThreads) k_scan_partial(const int32_t* in, int64_t n, int64_t* bsum) {
+  __shared__ int64_t red[kScanThreads / kWave];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanTile;
+  int64_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < kScanPer; j++) {
+    const int64_t i = b0 + j * kScanThreads + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  for (int k = kWave / 2; k > 0; k >>= 1) acc += __shfl_xor(acc, k, kWave);
+  if (lane_id() == 0) red[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < kScanThreads / kWave; w++) t += red[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+// In place: bsum[0..nb) becomes its exclusive scan, bsum[nb] the total (one workgroup).
+__global__ void __launch_bounds__(1024) k_scan_bsums(int64_t nb, int64_t* bsum) {
+  __shared__ int64_t wsum[1024 / kWave];
+  const int64_t per = (nb + 1023) / 1024;
   const int64_t a = (int64_t)threadIdx.x * per;
   const int64_t e = a + per < nb ? a + per : nb;
   int64_t sum = 0;
   for (int64_t b = a; b < e; b++) sum += bsum[b];
-  part[threadIdx.x] = sum;
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  const int64_t incl = wave_incl_scan(sum);
+  if (lane == kWave - 1) wsum[w] = incl;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t run = 0;
-    for (unsigned t = 0; t < blockDim.x; t++) {
-      const int64_t v = part[t];
-      part[t] = run;
-      run += v;
-    }
-    bsum[nb] = run;
+  if (w == 0) {
+    const int64_t v = lane < 1024 / kWave ? wsum[lane] : 0;
+    const int64_t vi = wave_incl_scan(v);
+    if (lane < 1024 / kWave) wsum[lane] = vi - v;
+    if (lane == 1024 / kWave - 1) bsum[nb] = vi;
   }
   __syncthreads();
-  int64_t run = part[threadIdx.x];
+  int64_t run = wsum[w] + incl - sum;
   for (int64_t b = a; b < e; b++) {
     const int64_t v = bsum[b];
     bsum[b] = run;
@@ -1469,22 +1292,44 @@ __global__ void k_scan_bsums(int64_t nb, int64_t* bsum) {
   }
 }
 
-__global__ void k_scan_final(const int32_t* in, int64_t n, const int64_t* bsum, int64_t nb, int64_t* out) {
-  __shared__ int64_t s[kScanBlock];
-  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
-  for (int t = threadIdx.x; t < kScanBlock; t += 256) s[t] = (b0 + t < n) ? in[b0 + t] : 0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t run = bsum[blockIdx.x];
-    for (int t = 0; t < kScanBlock; t++) {
-      const int64_t v = s[t];
-      s[t] = run;
-      run += v;
-    }
+__global__ void __launch_bounds__(kScanThreads) k_scan_final(const int32_t* in, int64_t n, const int64_t* bsum,
+                                                             int64_t nb, int64_t* out) {
+  __shared__ int64_t s[kScanTile + kScanTile / kScanPer];
+  __shared__ int64_t wsum[kScanThreads / kWave];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanTile;
+#pragma unroll
+  for (int j = 0; j < kScanPer; j++) {
+    const int idx = j * kScanThreads + threadIdx.x;
+    const int64_t i = b0 + idx;
+    s[idx + idx / kScanPer] = i < n ? in[i] : 0;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kScanBlock; t += 256)
-    if (b0 + t < n) out[b0 + t] = s[t];
+  const int base = threadIdx.x * (kScanPer + 1);  // padded position of the thread's 16 entries
+  int64_t x[kScanPer];
+  int64_t tot = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) {
+    x[k] = s[base + k];
+    tot += x[k];
+  }
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  const int64_t incl = wave_incl_scan(tot);
+  if (lane == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  int64_t run = bsum[blockIdx.x] + incl - tot;
+  for (int ww = 0; ww < w; ww++) run += wsum[ww];
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) {
+    s[base + k] = run;
+    run += x[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanPer; j++) {
+    const int idx = j * kScanThreads + threadIdx.x;
+    const int64_t i = b0 + idx;
+    if (i < n) out[i] = s[idx + idx / kScanPer];
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
 }
 
@@ -1841,12 +1686,16 @@ class Engine {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_));
     cus_ = prop.multiProcessorCount;
+    // add waves resident per CU: the LDS staging buffers, not registers, set the limit
+    int nb = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_add, kWave * kAddWaves, 0));
+    add_waves_per_cu_ = std::max(1, nb) * kAddWaves;
     const char* kt = std::getenv("SSQ_KERNEL_TIMING");
     timing_ = kt && kt[0] == '1';
     const char* tr = std::getenv("SSQ_TRACE_ROUNDS");  // per-round diagnostics on stderr
     trace_ = tr && tr[0] == '1';
     if (const char* e = std::getenv("SSQ_ROUNDS_PER_SYNC")) rounds_per_sync_ = std::max(1, std::min(8, std::atoi(e)));
-    if (const char* e = std::getenv("SSQ_TEST_NO_BLOCK_DUAL")) no_block_dual_ = e[0] == '1';
+    if (const char* e = std::getenv("SSQ_TEST_NO_SORT_DUAL")) no_sort_dual_ = e[0] == '1';
     // test hooks: smaller limits push short columns through the long-column paths
     if (const char* e = std::getenv("SSQ_TEST_SHORT_MAX")) short_max_ = std::max(2, std::min(kTile, std::atoi(e)));
     if (const char* e = std::getenv("SSQ_TEST_WINDOW_BITS"))
@@ -1888,7 +1737,7 @@ class Engine {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
                       &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
                       &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_,
-                      &b_wcnt_, &b_woff_, &b_scrc_, &b_scrr_, &b_kbase_})
+                      &b_scrc_, &b_kbase_, &b_skey_, &b_stmp_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
     if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
@@ -1927,8 +1776,8 @@ class Engine {
   int scan_counts(const int32_t* cnt, int64_t n, int64_t* out);
   int build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
                  const int32_t* dims, bool any_removed, int64_t** dcolptr, int32_t** drows, int32_t** ddims);
-  int dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows, bool any_removed,
-                      int64_t* dcolptr, int32_t* drows);
+  int dual_by_sort(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows, bool any_removed,
+                   int64_t* dcolptr, int32_t* drows);
 
   static constexpr int kTotPending = 0;   // [0..7] pending columns per dimension
   static constexpr int kTotPairs = 8;     // pairs
@@ -1942,14 +1791,15 @@ class Engine {
 
   int device_ = 0;
   int cus_ = 256;
+  int add_waves_per_cu_ = 12;
   hipStream_t stream_ = nullptr;
   uint32_t* host_ctr_ = nullptr;
   std::string err_;
   DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
       b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
-      b_drem_, b_scan_, b_pmeta_, b_prem_, b_wcnt_, b_woff_, b_scrc_, b_scrr_, b_kbase_;
+      b_drem_, b_scan_, b_pmeta_, b_prem_, b_scrc_, b_kbase_, b_skey_, b_stmp_;
   int64_t arena_hint_ = 0;
-  bool no_block_dual_ = false;  // test hook: always use the atomic anti-transpose
+  bool no_sort_dual_ = false;  // test hook: always use the atomic anti-transpose
   bool timing_ = false;
   bool trace_ = false;
   int rounds_per_sync_ = 2;  // add rounds launched between two reads of the worklist counters
@@ -2187,49 +2037,72 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
 
 
 int Engine::scan_counts(const int32_t* cnt, int64_t n, int64_t* out) {
-  const int64_t nb = std::max<int64_t>(1, (n + kScanBlock - 1) / kScanBlock);
+  const int64_t nb = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
   int64_t* bsum;
   int rc = ensure(b_scan_, (size_t)(nb + 1), &bsum);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)nb), dim3(256), 0, stream_, cnt, n, bsum);
+  hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)nb), dim3(kScanThreads), 0, stream_, cnt, n, bsum);
   hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, stream_, nb, bsum);
-  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(256), 0, stream_, cnt, n, bsum, nb, out);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), 0, stream_, cnt, n, bsum, nb, out);
   HIPCHK(hipGetLastError());
   return SSQ_OK;
 }
 
-// Anti-transpose by row windows (see k_win_place).  Returns 1 when it produced the dual, 0
-// when some window holds more than kWinCap entries (the general path is used then).
-int Engine::dual_by_windows(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows,
-                            bool any_removed, int64_t* dcolptr, int32_t* drows) {
+// Stable LSD radix sort of (uint32 key, int32 value) pairs over key bits [0, end_bit): rocPRIM's
+// onesweep with R-bit digits per pass.
+template <unsigned R>
+static hipError_t sort_pairs_r(void* tmp, size_t& bytes, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
+                               int32_t* vout, int64_t count, int end_bit, hipStream_t s) {
+  using Onesweep = rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>,
+                                                       R, rocprim::block_radix_rank_algorithm::match>;
+  using Cfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config, Onesweep>;
+  return rocprim::radix_sort_pairs<Cfg>(tmp, bytes, kin, kout, vin, vout, (size_t)count, 0u, (unsigned)end_bit, s);
+}
+// 9-bit digits where they save a pass over 8-bit ones (keys of 17-18 or 25-27 bits: 3 passes
+// instead of 4 on 124M pairs is 0.9 ms less); 8-bit passes are the faster ones otherwise.
+static hipError_t sort_pairs(void* tmp, size_t& bytes, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
+                             int32_t* vout, int64_t count, int end_bit, hipStream_t s) {
+  if ((end_bit + 8) / 9 < (end_bit + 7) / 8)
+    return sort_pairs_r<9>(tmp, bytes, kin, kout, vin, vout, count, end_bit, s);
+  return sort_pairs_r<8>(tmp, bytes, kin, kout, vin, vout, count, end_bit, s);
+}
+
+// Anti-transpose by a stable radix sort of (n-1-r, n-1-c) pairs over the key's bits; the
+// sorted values are the dual's rows, the runs of equal keys its column lengths.
+int Engine::dual_by_sort(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows,
+                         bool any_removed, int64_t* dcolptr, int32_t* drows) {
   const uint8_t* rem = any_removed ? (const uint8_t*)d.removed : nullptr;
-  const int64_t nw = (n + kWinRows - 1) / kWinRows;
-  uint32_t* wc;
-  int64_t* woff;
-  int32_t* scr_c;
-  uint8_t* scr_r;
+  uint32_t* keys;
+  int32_t* vals;
+  int32_t* len;
   int rc;
-  if ((rc = ensure(b_wcnt_, (size_t)(2 * nw + 16), &wc)) || (rc = ensure(b_woff_, (size_t)(nw + 1), &woff)) ||
-      (rc = ensure(b_scrc_, (size_t)std::max<int64_t>(nnz, 1), &scr_c)) ||
-      (rc = ensure(b_scrr_, (size_t)std::max<int64_t>(nnz, 1), &scr_r)))
-    return -rc;
-  uint32_t* wfill = wc + nw;
-  uint32_t* flag = wc + 2 * nw;
-  HIPCHK(hipMemsetAsync(wc, 0, sizeof(uint32_t) * (2 * nw + 16), stream_));
-  const int chunks = (int)std::min<int64_t>((n + kChunkCols - 1) / kChunkCols, (int64_t)cus_ * 8);
-  hipLaunchKernelGGL(k_win_count, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows, rem,
-                     wc);
-  rc = scan_counts((const int32_t*)wc, nw, woff);
-  if (rc) return -rc;
-  hipLaunchKernelGGL(k_win_bucket, dim3((unsigned)std::max(1, chunks)), dim3(256), 0, stream_, n, colptr, rows, rem,
-                     (const int64_t*)woff, wfill, scr_c, scr_r);
-  const int64_t pg = std::min<int64_t>((nw + kPlaceWaves - 1) / kPlaceWaves, (int64_t)cus_ * 16);
-  hipLaunchKernelGGL(k_win_place, dim3((unsigned)std::max<int64_t>(1, pg)), dim3(kWave * kPlaceWaves), 0, stream_, n,
-                     nw, (const int64_t*)woff, (const int32_t*)scr_c, (const uint8_t*)scr_r, dcolptr, drows, flag);
+  const int64_t m = std::max<int64_t>(nnz, 1);
+  if ((rc = ensure(b_skey_, (size_t)(2 * m), &keys)) || (rc = ensure(b_scrc_, (size_t)m, &vals)) ||
+      (rc = ensure(b_misc_, (size_t)n, &len)))
+    return rc;
+  uint32_t* keys_out = keys + m;
+  int end_bit = 1;
+  while (end_bit < 32 && ((int64_t)1 << end_bit) <= n) end_bit++;  // keys lie in [0, n]
+  size_t tb = 0;
+  if (sort_pairs(nullptr, tb, keys, keys_out, vals, drows, nnz, end_bit, stream_) != hipSuccess) {
+    err_ = "radix sort sizing failed";
+    return SSQ_EDEVICE;
+  }
+  uint8_t* tmp;
+  if ((rc = ensure(b_stmp_, tb, &tmp))) return rc;
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_dual_keys, dim3(g), dim3(256), 0, stream_, n, nnz, colptr, rows, rem, keys, vals);
   HIPCHK(hipGetLastError());
-  rc = sync_read(flag, 1);
-  if (rc) return -rc;
-  return host_ctr_[0] ? 0 : 1;
+  if (nnz > 0 && sort_pairs(tmp, tb, keys, keys_out, vals, drows, nnz, end_bit, stream_) != hipSuccess) {
+    err_ = "radix sort failed";
+    return SSQ_EDEVICE;
+  }
+  HIPCHK(hipMemsetAsync(len, 0, sizeof(int32_t) * n, stream_));
+  const int ge = grid_for((nnz + 3) / 4);
+  hipLaunchKernelGGL(k_run_first, dim3(ge), dim3(256), 0, stream_, n, nnz, keys_out, dcolptr);
+  hipLaunchKernelGGL(k_run_len, dim3(ge), dim3(256), 0, stream_, n, nnz, keys_out, dcolptr, len);
+  HIPCHK(hipGetLastError());
+  return scan_counts(len, n, dcolptr);  // run starts are overwritten by the column pointers
 }
 
 int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
@@ -2242,12 +2115,13 @@ int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int
       (rc = ensure(b_misc_, (size_t)n, &cnt)))
     return rc;
   const int g = grid_for(n);
-  int by_windows = 0;
-  if (n >= 1 && !no_block_dual_) {
-    by_windows = dual_by_windows(d, n, nnz, colptr, rows, any_removed, *dcolptr, *drows);
-    if (by_windows < 0) return -by_windows;
+  bool sorted = false;
+  if (!no_sort_dual_) {
+    rc = dual_by_sort(d, n, nnz, colptr, rows, any_removed, *dcolptr, *drows);
+    if (rc == SSQ_OK) sorted = true;
+    else if (rc != SSQ_ENOMEM) return rc;  // no room for the sort's buffers: the atomic path
   }
-  if (by_windows) {
+  if (sorted) {
     hipLaunchKernelGGL(k_dual_dims, dim3(g), dim3(256), 0, stream_, n, dims, maxdim, d.removed, *ddims, drem);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(d.removed, drem, (size_t)n, hipMemcpyDeviceToDevice, stream_));
@@ -2430,7 +2304,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     if ((rc = ensure(b_work_, (size_t)(5 * std::max<int64_t>(upend, 1) + 64), &W0))) return rc;
     const int64_t wcap = std::max<int64_t>(upend, 1);
     for (int k = 0; k < 5; k++) d.work[k] = W0 + k * wcap;
-    const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * 12);
+    const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * add_waves_per_cu_);
     const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
     const int rounds_per_sync = rounds_per_sync_;
     const int long_grid = cus_ * 2;

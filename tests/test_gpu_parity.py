@@ -283,14 +283,17 @@ def test_batch_reduce_points_equals_alone(h, maxdim):
             assert np.array_equal(pairs, O.twist_pairs(mo))
 
 
-@pytest.mark.parametrize("general", [False, True])
-def test_dual_window_transpose_matches_general(monkeypatch, general):
-    """Cohomology through the row-window anti-transpose and through the atomic one (forced by
-    the test hook) give the oracle's pairs, batched and device-resident."""
+DUAL_PATHS = {"sort": {}, "atomic": {"SSQ_TEST_NO_SORT_DUAL": "1"}}
+
+
+@pytest.mark.parametrize("path", list(DUAL_PATHS))
+def test_dual_transpose_paths(monkeypatch, path):
+    """Cohomology through each anti-transpose (radix sort, the default; atomic counts and
+    fills, forced by the test hook) gives the oracle's pairs, batched and device-resident."""
     from scipy.spatial.distance import pdist
     from specseqplusplus_amd import configs as CF
-    if general:
-        monkeypatch.setenv("SSQ_TEST_NO_BLOCK_DUAL", "1")
+    for k, v in DUAL_PATHS[path].items():
+        monkeypatch.setenv(k, v)
     hd = S.Handle(0)
     try:
         rng = np.random.default_rng(77)
@@ -303,7 +306,7 @@ def test_dual_window_transpose_matches_general(monkeypatch, general):
         outs = hd.batch_reduce(mats, p)
         for m, (pairs, ess) in zip(mats, outs):
             mo = O.Matrix(m.colptr, m.rows, m.dims)
-            assert np.array_equal(pairs, O.twist_pairs(mo)), (general, m.n)
+            assert np.array_equal(pairs, O.twist_pairs(mo)), (path, m.n)
         # the same batch resident in HBM (the path of a device-resident caller)
         from specseqplusplus_amd.autotune import block_diagonal
         bd, base = block_diagonal(mats)
@@ -316,10 +319,12 @@ def test_dual_window_transpose_matches_general(monkeypatch, general):
         hd.close()
 
 
-def test_dual_window_overflow_falls_back():
-    """A vertex on 3000 edges puts 3000 entries into one 16-row window of the anti-transpose
-    (more than a window holds in LDS): the engine must fall back to the atomic path and stay
-    exact, alone and next to ordinary complexes in a batch."""
+@pytest.mark.parametrize("path", list(DUAL_PATHS))
+def test_dual_long_coface_row(monkeypatch, path):
+    """A vertex on 3000 edges: one 3000-entry dual column next to short ones; each
+    anti-transpose stays exact, alone and next to ordinary complexes in a batch."""
+    for k, v in DUAL_PATHS[path].items():
+        monkeypatch.setenv(k, v)
     k = 3000
     nv = k + 1
     colptr = [0] * (nv + 1)

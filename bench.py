@@ -234,7 +234,7 @@ def main():
     ap.add_argument("--single", choices=["c1", "c2", "c3"],
                     help="time one single-complex configuration instead of the batched stream")
     ap.add_argument("--dualize", action="store_true", help="with --single: reduce in cohomology")
-    ap.add_argument("--streams", type=int, default=2,
+    ap.add_argument("--streams", type=int, default=3,
                     help="concurrent groups (one HIP stream each) the batched stream is split into")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default batched parameters instead of tuning each size class")
@@ -267,7 +267,6 @@ def main():
     # each size class is split into interleaved groups; every group is one block-diagonal
     # batch on its own handle (HIP stream), and a step reduces all groups concurrently
     total_cells = sum(m.n for m in mats)
-    plans = []
     class_params = []
     for ci, idx in enumerate(classes):
         if args.no_tune:
@@ -275,6 +274,15 @@ def main():
         else:
             chosen, table = tune(h, [mats[i] for i in idx[:min(len(idx), 96)]])
         class_params.append((chosen, table))
+    # Each handle owns one HIP stream, and HIP puts a stream on one of GPU_MAX_HW_QUEUES hardware
+    # queues (4 by default) when it is created: a stream created while four others are open
+    # shares a queue and serialises with another busy stream, even after the others close
+    # (4 groups: 32.6 ms/step created with 4 streams open, 46.6 ms with 5).  So every leg
+    # closes its handles before the next one opens its own.
+    h.close()
+    plans = []
+    for ci, idx in enumerate(classes):
+        chosen, table = class_params[ci]
         cells = sum(mats[i].n for i in idx)
         parts = max(1, int(round(args.streams * cells / total_cells)))
         for g in range(parts):
@@ -284,9 +292,9 @@ def main():
             plans.append({"idx": gidx, "h": hg, "dm": hg.upload(bd), "out": hg.device_empty(2 * bd.n, np.int64),
                           "p": S.params(**chosen), "chosen": chosen, "table": table, "base": cbase,
                           "cells": bd.n, "nnz": bd.nnz, "cls": ci})
+            del bd
         print(f"[bench] class {ci}: {len(idx)} complexes, {cells} cells in {parts} group(s), tuned {chosen} "
               f"({[t['ms'] for t in table]} ms)", file=sys.stderr, flush=True)
-    del bd
     for pl in plans:
         pl["h"].synchronize()
     pool = ThreadPoolExecutor(len(plans))
@@ -316,6 +324,12 @@ def main():
     class_of = np.zeros(args.complexes, int)
     for pl in plans:
         class_of[pl["idx"]] = pl["cls"]
+    pool.shutdown()
+    for pl in plans:
+        for a in (pl["dm"].colptr, pl["dm"].rows, pl["dm"].dims, pl["out"]):
+            a.free()
+        pl["h"].close()
+    h = S.Handle(0) if args.plain > 0 or args.served > 0 else None
 
     # parity sample against the oracle (CPU); also the CPU baseline rate
     parity = {"checked": 0, "bit_exact": True}
@@ -379,6 +393,7 @@ def main():
             h.reduce_resident(dm, d_pairs, pp)
         h.synchronize()
         plain = len(dev_mats) / (time.perf_counter() - tt)
+        del dev_mats, d_pairs
         print(f"[bench] plain per-complex GPU: {plain:.0f} diagrams/s", file=sys.stderr, flush=True)
 
     # served stream: Poisson arrivals; half point clouds (64-4096 points, Rips built by the
@@ -410,6 +425,7 @@ def main():
         served["plain_per_request_diagrams_per_s"] = round(n_plain / (time.perf_counter() - tt), 1)
         print(f"[bench] served mix, plain one request at a time: "
               f"{served['plain_per_request_diagrams_per_s']} diagrams/s", file=sys.stderr, flush=True)
+        h.close()  # the server's workers open their own streams
         for qps in args.qps:
             rng = np.random.default_rng(int(qps))
             kinds = rng.integers(0, 2, args.served)

@@ -5,7 +5,7 @@
 //                collapser, coface propagation on device            (specseq.h:410-438,
 //                                                                     kernel.h:190-198)
 //   model bit 0  high-dimension guided clearing of max-dim cells      (kernel.h:121-292)
-//   dualize      the anti-transpose by 16-row windows (cohomology);
+//   dualize      the anti-transpose by a stable radix sort (cohomology);
 //                pairs map back as (b', d') -> (n-1-d', n-1-b')       (specseq.h:303-363,
 //                                                                     kernel.h:521-580)
 //   init passes  low / leftmost-one, apparent pairs, clearing, compression

@@ -620,6 +620,7 @@ int64_t or_apply_collapse(int64_t n, const int64_t* colptr, const int32_t* rows,
                           const int32_t* dims, uint8_t* skip) {
   int64_t ne = 0;
   int32_t maxd = 0;
+  if (n < 0) return -1;
   for (int64_t c = 0; c < n; c++) {
     if (dims[c] == 1) {
       if (colptr[c + 1] - colptr[c] != 2) return -1;
@@ -627,7 +628,7 @@ int64_t or_apply_collapse(int64_t n, const int64_t* colptr, const int32_t* rows,
     }
     if (dims[c] > maxd) maxd = dims[c];
   }
-  memset(skip, 0, (size_t)n);
+  if (n > 0) memset(skip, 0, (size_t)n);
   if (ne == 0) return 0;
   int32_t* u = (int32_t*)malloc(sizeof(int32_t) * (size_t)ne);
   int32_t* v = (int32_t*)malloc(sizeof(int32_t) * (size_t)ne);

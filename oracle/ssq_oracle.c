@@ -250,6 +250,7 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
                         const int32_t* dims, const uint8_t* skip, int64_t* pairs_out,
                         double* stats_out) {
   int32_t maxd = 0;
+  if (n < 0) return -1;
   for (int64_t c = 0; c < n; c++)
     if (dims[c] > maxd) maxd = dims[c];
   int32_t* pivot_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1)); /* row -> col */
@@ -323,7 +324,7 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
 /* ------------------------------------------------------------------------------------ */
 int64_t or_dense_standard(int64_t n, const int64_t* colptr, const int32_t* rows,
                           const uint8_t* skip, int64_t* pairs_out) {
-  if (n > 65536) return -1;
+  if (n > 65536 || n < 0) return -1;
   int64_t W = (n + 63) / 64;
   if (W < 1) W = 1;
   uint64_t* M = (uint64_t*)calloc((size_t)(n ? n : 1) * (size_t)W, 8);

@@ -1802,7 +1802,7 @@ class Engine {
   bool no_sort_dual_ = false;  // test hook: always use the atomic anti-transpose
   bool timing_ = false;
   bool trace_ = false;
-  int rounds_per_sync_ = 2;  // add rounds launched between two reads of the worklist counters
+  int rounds_per_sync_ = 3;  // add rounds launched between two reads of the worklist counters
   int32_t short_max_ = kTile;
   int32_t long_window_ = kLongBits;
   hipEvent_t ev_[32] = {};  // 4 per round, up to 8 rounds per sync

@@ -218,8 +218,8 @@ def single_config(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--complexes", type=int, default=2048)
     ap.add_argument("--nmin", type=int, default=64)
     ap.add_argument("--nmax", type=int, default=256)

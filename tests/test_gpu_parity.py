@@ -355,3 +355,25 @@ def test_dual_long_coface_row(monkeypatch, path):
         assert np.array_equal(outs[0][0], outs[2][0])
     finally:
         hd.close()
+
+
+@pytest.mark.parametrize("n", [4095, 4096, 4097, 8191, 8193, 12289])
+def test_cell_counts_at_scan_tile_boundaries(h, n):
+    """Exactly n cells around multiples of the 4096-count scan tile (the column pointers of the
+    anti-transpose and the compactions scan over n): a path on v vertices and isolated
+    vertices to pad; homology and cohomology against the oracle."""
+    v = (n + 1) // 2
+    pad = n - (2 * v - 1)
+    colptr, rows, dims = [0], [], []
+    for _ in range(v + pad):  # vertices 0..v-1 on the path, then the isolated ones
+        colptr.append(len(rows))
+        dims.append(0)
+    edge0 = v + pad
+    for i in range(v - 1):  # edge (i, i+1) is column edge0 + i
+        rows += [i, i + 1]
+        colptr.append(len(rows))
+        dims.append(1)
+    mo = O.Matrix(np.array(colptr, np.int64), np.array(rows, np.int32), np.array(dims, np.int32))
+    assert mo.n == n
+    for kw in [{}, {"dualize": 1}]:
+        check(h, mo, **kw)

@@ -17,6 +17,7 @@
 // neighbourhood.  It does not depend on vertex labels, so vertices are compacted.
 #include <algorithm>
 #include <atomic>
+#include <memory>
 #include <queue>
 #include <thread>
 #include <vector>
@@ -33,7 +34,8 @@ struct Graph {
   // of (a, x) and (b, x) -- the edge's triangles.  The valid common neighbours of a test are
   // a filter of this list, not a merge of two adjacency lists.  Empty when too large.
   std::vector<int64_t> toff;
-  std::vector<int32_t> tx, tea, teb, tcnt;  // edge e's triangles: [toff[e], toff[e] + tcnt[e])
+  std::unique_ptr<int32_t[]> tx, tea, teb;  // edge e's triangles: [toff[e], toff[e] + tcnt[e])
+  std::vector<int32_t> tcnt;
   std::vector<int32_t> eu, ev;
 };
 constexpr int64_t kTriListMax = int64_t(1) << 28;  // entries (12 bytes each)
@@ -162,26 +164,16 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
       p = fill[ev[i]]++;
       g.nbr[p] = eu[i]; g.eid[p] = (int32_t)i;
     }
-    std::vector<std::pair<int32_t, int32_t>> tmp;
-    for (int64_t a = 0; a < nv; a++) {
-      tmp.clear();
-      for (int64_t p = g.off[a]; p < g.off[a + 1]; p++) tmp.emplace_back(g.nbr[p], g.eid[p]);
-      std::sort(tmp.begin(), tmp.end());
-      for (int64_t p = g.off[a], k = 0; p < g.off[a + 1]; p++, k++) { g.nbr[p] = tmp[k].first; g.eid[p] = tmp[k].second; }
-    }
   }
-  int64_t maxdeg = 1;
-  for (int64_t a = 0; a < nv; a++) maxdeg = std::max(maxdeg, g.off[a + 1] - g.off[a]);
-  g.eu = eu;
-  g.ev = ev;
   if (threads < 1) threads = 1;
-  auto parallel_edges = [&](auto&& body) {  // body(e) for every edge, on all threads
+  // body(i) for i in [0, count), chunks of `chunk` taken by all threads
+  auto parallel_range = [&](int64_t count, int64_t chunk, auto&& body) {
     std::atomic<int64_t> next{0};
     auto work = [&]() {
       for (;;) {
-        const int64_t s0 = next.fetch_add(1024);
-        if (s0 >= ne) break;
-        for (int64_t e = s0; e < std::min(ne, s0 + 1024); e++) body(e);
+        const int64_t s0 = next.fetch_add(chunk);
+        if (s0 >= count) break;
+        for (int64_t i = s0; i < std::min(count, s0 + chunk); i++) body(i);
       }
     };
     std::vector<std::thread> pool;
@@ -189,19 +181,33 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
     work();
     for (auto& t : pool) t.join();
   };
+  auto parallel_edges = [&](auto&& body) { parallel_range(ne, 1024, body); };
+  // each vertex's neighbours ascending (they were filled in edge order)
+  parallel_range(nv, 16, [&](int64_t a) {
+    thread_local std::vector<std::pair<int32_t, int32_t>> tmp;
+    tmp.clear();
+    for (int64_t p = g.off[a]; p < g.off[a + 1]; p++) tmp.emplace_back(g.nbr[p], g.eid[p]);
+    std::sort(tmp.begin(), tmp.end());
+    for (int64_t p = g.off[a], k = 0; p < g.off[a + 1]; p++, k++) { g.nbr[p] = tmp[k].first; g.eid[p] = tmp[k].second; }
+  });
+  int64_t maxdeg = 1;
+  for (int64_t a = 0; a < nv; a++) maxdeg = std::max(maxdeg, g.off[a + 1] - g.off[a]);
+  g.eu = eu;
+  g.ev = ev;
   {  // triangle lists of every edge, in one pass: edge e gets min(deg a, deg b) slots
     std::vector<int64_t> cap((size_t)ne + 1, 0);
     for (int64_t e = 0; e < ne; e++)
       cap[e + 1] = cap[e] + std::min(g.off[eu[e] + 1] - g.off[eu[e]], g.off[ev[e] + 1] - g.off[ev[e]]);
     if (cap[ne] <= kTriListMax) {
-      g.tx.resize((size_t)cap[ne]);
-      g.tea.resize((size_t)cap[ne]);
-      g.teb.resize((size_t)cap[ne]);
+      // filled below on all threads; a vector would zero them first on one thread
+      g.tx.reset(new int32_t[(size_t)cap[ne]]);
+      g.tea.reset(new int32_t[(size_t)cap[ne]]);
+      g.teb.reset(new int32_t[(size_t)cap[ne]]);
       g.tcnt.resize((size_t)ne);
       parallel_edges([&](int64_t e) {
         const int64_t o = cap[e];
-        g.tcnt[e] = (int32_t)common(g, eu[e], ev[e], [](int32_t) { return true; }, g.tx.data() + o,
-                                    g.tea.data() + o, g.teb.data() + o);
+        g.tcnt[e] = (int32_t)common(g, eu[e], ev[e], [](int32_t) { return true; }, g.tx.get() + o,
+                                    g.tea.get() + o, g.teb.get() + o);
       });
       g.toff.swap(cap);
     }

@@ -377,3 +377,24 @@ def test_cell_counts_at_scan_tile_boundaries(h, n):
     assert mo.n == n
     for kw in [{}, {"dualize": 1}]:
         check(h, mo, **kw)
+
+
+def test_dual_sort_nine_bit_digits(h):
+    """A dualized batch of 2^16..2^18 cells: its sort keys have 17-18 bits, where the
+    anti-transpose sorts with 9-bit digits (two passes instead of three 8-bit ones).  Every
+    complex's pairs equal the oracle's."""
+    from scipy.spatial.distance import pdist
+    from specseqplusplus_amd import configs as CF
+    rng = np.random.default_rng(2718)
+    mats, total, i = [], 0, 0
+    while total < 100_000:
+        X = CF.cloud(rng, i % 3, 90 + (i % 5) * 10).astype(np.float32)
+        r = float(np.percentile(pdist(X.astype(np.float64)), 30))
+        m = S.vr_build(X, r, 2)
+        mats.append(m)
+        total += m.n
+        i += 1
+    assert (1 << 16) <= total < (1 << 18)
+    outs = h.batch_reduce(mats, S.params(dualize=1, ss_block_size=1 << 30))
+    for m, (pairs, ess) in zip(mats, outs):
+        assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))

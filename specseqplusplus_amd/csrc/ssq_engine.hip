@@ -880,8 +880,9 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
         __syncthreads();
         r_len = sh_count;
         // new low and the row below it: the two highest set bits, scanning down from the old
-        // low's word.  Wavefront 0 alone scans, 64 words per step, and publishes them; the
-        // second is looked for up to 8 steps past the first.  Two barriers per addition:
+        // low's word.  Wavefront 0 alone scans, 64 words per step (one ballot finds the
+        // nonzero words), and publishes them; the second is looked for up to 8 steps past the
+        // first.  Two barriers per addition:
         // every read of this step's LDS values happens before the second, every write of the
         // next step after it.
         if (tid < kWave) {
@@ -889,26 +890,22 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
           int32_t t1 = -1, t2 = -1, past = 0;
           int32_t base = r_len > 0 ? (l - lo) >> 5 : -1;  // r_len is uniform (read from LDS)
           for (; base >= 0; base -= kWave) {  // t1, t2 are uniform in the wavefront: so are the exits
+            // lane k holds word base - k, so rows descend with the lane: the highest rows are
+            // in the first lanes whose words are nonzero, read one word at a time (scalar)
             const int32_t w = base - lane;
-            int32_t x1 = -1, x2 = -1;
-            if (w >= 0) {
-              uint32_t m = bits[w];
-              if (m) {
-                const int h = 31 - __clz(m);
-                x1 = lo + (w << 5) + h;
-                m &= ~(1u << h);
-                if (m) x2 = lo + (w << 5) + (31 - __clz(m));
+            const uint32_t m = w >= 0 ? bits[w] : 0u;
+            uint64_t nz = __ballot(m != 0);
+            while (nz && t2 < 0) {
+              const int fl = __ffsll((unsigned long long)nz) - 1;
+              nz &= nz - 1;
+              uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, fl);
+              const int32_t row0 = lo + ((base - fl) << 5);
+              while (mm && t2 < 0) {
+                const int h = 31 - __clz(mm);
+                mm &= ~(1u << h);
+                if (t1 < 0) t1 = row0 + h;
+                else t2 = row0 + h;
               }
-            }
-            for (int k = kWave / 2; k > 0; k >>= 1) {
-              const int32_t o1 = __shfl_xor(x1, k, kWave), o2 = __shfl_xor(x2, k, kWave);
-              top2_merge(x1, x2, o1, o2);
-            }
-            if (t1 < 0) {
-              t1 = x1;
-              t2 = x2;
-            } else {
-              t2 = x1;
             }
             if (t2 >= 0 || (t1 >= 0 && r_len == 1)) break;
             if (t1 >= 0 && ++past >= 8) break;

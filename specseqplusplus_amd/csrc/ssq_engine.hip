@@ -875,8 +875,14 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
               delta += (old & m) ? -1 : 1;
             }
         }
-        delta = wave_incl_scan(delta);
-        if (lane_id() == kWave - 1) atomicAdd(&sh_count, delta);
+        {  // the wavefront's sum of the changes by bit planes: |delta| <= 1024 (a pivot in the
+           // bit set has at most kLongBits rows over kLongThreads threads), offset to >= 0
+          const uint32_t u = (uint32_t)(delta + 2048);
+          uint32_t sum = 0;
+#pragma unroll
+          for (int b = 0; b < 12; b++) sum += (uint32_t)__popcll(__ballot((u >> b) & 1u)) << b;
+          if (lane_id() == 0) atomicAdd(&sh_count, (int32_t)sum - 2048 * kWave);
+        }
         __syncthreads();
         r_len = sh_count;
         // new low and the row below it: the two highest set bits, scanning down from the old

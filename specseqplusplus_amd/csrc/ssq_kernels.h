@@ -105,9 +105,17 @@ __device__ __forceinline__ int wave_xor_lds(const int32_t* A, int a, const int32
   const int i0 = split_adj(A, a, B, b, k0, &j0);
   const int i1 = split_adj(A, a, B, b, k1, &j1);
   const int cnt = seg_xor<false>(A, i0, i1, B, j0, j1, nullptr);
-  const int incl = wave_incl_scan(cnt);
-  const int r = wave_bcast(incl, kWave - 1);
-  seg_xor<true>(A, i0, i1, B, j0, j1, O + (incl - cnt));
+  // each lane's output offset and the total, by bit planes of the counts (cnt <= seg, and seg
+  // is the same in every lane): one ballot per bit, the lanes below counted by mbcnt
+  int excl = 0, r = 0;
+  const int nbits = seg > 0 ? 32 - __clz(seg) : 0;
+  for (int bit = 0; bit < nbits; bit++) {
+    const uint64_t m = __ballot((cnt >> bit) & 1);
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    excl += below << bit;
+    r += __popcll(m) << bit;
+  }
+  seg_xor<true>(A, i0, i1, B, j0, j1, O + excl);
   wave_sync();
   return r;
 }

@@ -225,7 +225,7 @@ def main():
     ap.add_argument("--nmax", type=int, default=256)
     ap.add_argument("--plain", type=int, default=512, help="complexes timed one call each (plain GPU)")
     ap.add_argument("--served", type=int, default=2000, help="served requests per offered load (0 = skip)")
-    ap.add_argument("--qps", type=float, nargs="+", default=[200.0, 1000.0, 2000.0, 4000.0])
+    ap.add_argument("--qps", type=float, nargs="+", default=[200.0, 1000.0, 2000.0, 2500.0, 4000.0])
     ap.add_argument("--server-streams", type=int, default=4)
     ap.add_argument("--server-cells", type=int, default=2_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -99,11 +99,15 @@ __device__ __forceinline__ int wave_xor_lds(const int32_t* A, int a, const int32
   const int seg = (total + kWave - 1) / kWave;
   int k0 = lane * seg;
   if (k0 > total) k0 = total;
-  int k1 = k0 + seg;
-  if (k1 > total) k1 = total;
-  int j0, j1;
+  // a lane's segment ends where the next lane's begins (k1 of lane i is k0 of lane i + 1, and
+  // the split is a function of the position): one search per lane, the end from the neighbour
+  int j0;
   const int i0 = split_adj(A, a, B, b, k0, &j0);
-  const int i1 = split_adj(A, a, B, b, k1, &j1);
+  int i1 = __shfl_down(i0, 1, kWave), j1 = __shfl_down(j0, 1, kWave);
+  if (lane == kWave - 1) {  // k1 == total here: the split after everything
+    i1 = a;
+    j1 = b;
+  }
   const int cnt = seg_xor<false>(A, i0, i1, B, j0, j1, nullptr);
   // each lane's output offset and the total, by bit planes of the counts (cnt <= seg, and seg
   // is the same in every lane): one ballot per bit, the lanes below counted by mbcnt

@@ -39,8 +39,9 @@ def lib():
         for f in ("or_dense_standard", "or_rank_pairs"):
             getattr(L, f).argtypes = [ctypes.c_int64, _i64p, _i32p, _u8p, _i64p]
             getattr(L, f).restype = ctypes.c_int64
-        L.or_twist_reduce.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, _u8p, _i64p, _dp]
-        L.or_twist_reduce.restype = ctypes.c_int64
+        for f in ("or_twist_reduce", "or_twist_reduce_dual"):
+            getattr(L, f).argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, _u8p, _i64p, _dp]
+            getattr(L, f).restype = ctypes.c_int64
         L.or_collapse_edges.argtypes = [ctypes.c_int64, _i32p, _i32p, _u8p]
         L.or_apply_collapse.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, _u8p]
         L.or_apply_collapse.restype = ctypes.c_int64
@@ -121,13 +122,14 @@ def _skip(m, skip):
     return s, _p(s, _u8p)
 
 
-def twist_pairs(m, skip=None, stats=False):
-    """Persistence pairs (np x 2 int64, sorted by birth) by twist reduction."""
+def twist_pairs(m, skip=None, stats=False, dual=False):
+    """Persistence pairs (np x 2 int64, sorted by birth) by twist reduction; dual=True reduces
+    the anti-transpose (cohomology) and maps the pairs back."""
     out = np.zeros((max(m.n, 1), 2), dtype=np.int64)
     st = np.zeros(3, dtype=np.float64)
     s, sp = _skip(m, skip)
-    k = lib().or_twist_reduce(m.n, _p(m.colptr, _i64p), _p(m.rows, _i32p), _p(m.dims, _i32p),
-                              sp, _p(out, _i64p), _p(st, _dp))
+    f = lib().or_twist_reduce_dual if dual else lib().or_twist_reduce
+    k = f(m.n, _p(m.colptr, _i64p), _p(m.rows, _i32p), _p(m.dims, _i32p), sp, _p(out, _i64p), _p(st, _dp))
     pairs = out[:k].copy()
     if stats:
         return pairs, {"additions": float(st[0]), "bytes": float(st[1]), "max_chain": float(st[2])}
@@ -166,23 +168,6 @@ def ref_collapse_edges(u, v):
     kept = np.zeros(len(u), dtype=np.uint8)
     ref().ref_collapse_edges(len(u), _p(u, _i32p), _p(v, _i32p), _p(kept, _u8p))
     return kept
-
-
-def dual_matrix(m):
-    """Anti-transpose (cohomology): cell c -> n-1-c, dimension maxdim - dim."""
-    n = m.n
-    cols = [[] for _ in range(n)]
-    for c in range(n - 1, -1, -1):
-        for r in m.rows[m.colptr[c]:m.colptr[c + 1]]:
-            cols[n - 1 - int(r)].append(n - 1 - c)
-    maxd = int(m.dims.max()) if n else 0
-    dims = (maxd - m.dims)[::-1]
-    return Matrix.from_columns(cols, dims)
-
-
-def dual_pairs_to_primal(pairs, n):
-    out = np.stack([n - 1 - pairs[:, 1], n - 1 - pairs[:, 0]], 1) if len(pairs) else pairs
-    return out[np.argsort(out[:, 0])] if len(out) else out
 
 
 def collapse_skip(m):

@@ -321,6 +321,59 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
 
 /* ------------------------------------------------------------------------------------ */
 /* Standard algorithm on dense bit columns: left to right, no clearing, no dim order.      */
+/* Cohomology: the same twist reduction on the anti-transpose, pairs mapped back
+ * (specseq.h:303-363 builds the anti-transpose when dualize is set; specseq.h:1188 maps a dual
+ * pair (b', d') to (n-1-d', n-1-b')).  Dual cell n-1-c has the coboundary of c as its column,
+ * rows ascending, and dimension maxdim - dim(c).  Skipped cells and their entries are left out
+ * of the dual. */
+int64_t or_twist_reduce_dual(int64_t n, const int64_t* colptr, const int32_t* rows,
+                             const int32_t* dims, const uint8_t* skip, int64_t* pairs_out,
+                             double* stats_out) {
+  if (n < 0) return -1;
+  int32_t maxd = 0;
+  for (int64_t c = 0; c < n; c++)
+    if (dims[c] > maxd) maxd = dims[c];
+  const int64_t nnz = colptr[n];
+  int64_t* dcp = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+  int32_t* drw = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+  int32_t* ddm = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  uint8_t* dsk = skip ? (uint8_t*)malloc((size_t)(n ? n : 1)) : NULL;
+  int64_t* dob = (int64_t*)malloc(sizeof(int64_t) * 2 * (size_t)(n ? n : 1));
+  int32_t* death = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  /* counts: entry (r, c) of the primal is entry (n-1-c, n-1-r) of the dual */
+  for (int64_t c = 0; c < n; c++) {
+    if (skip && skip[c]) continue;
+    for (int64_t k = colptr[c]; k < colptr[c + 1]; k++) dcp[n - 1 - rows[k] + 1]++;
+  }
+  for (int64_t i = 0; i < n; i++) dcp[i + 1] += dcp[i];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  for (int64_t i = 0; i < n; i++) fill[i] = dcp[i];
+  /* primal columns from the last to the first: each dual column's rows come out ascending */
+  for (int64_t c = n - 1; c >= 0; c--) {
+    if (skip && skip[c]) continue;
+    for (int64_t k = colptr[c]; k < colptr[c + 1]; k++) {
+      const int64_t dc = n - 1 - rows[k];
+      drw[fill[dc]++] = (int32_t)(n - 1 - c);
+    }
+  }
+  for (int64_t c = 0; c < n; c++) {
+    ddm[n - 1 - c] = maxd - dims[c];
+    if (dsk) dsk[n - 1 - c] = skip[c];
+  }
+  const int64_t np = or_twist_reduce(n, dcp, drw, ddm, dsk, dob, stats_out);
+  for (int64_t i = 0; i < n; i++) death[i] = -1;
+  for (int64_t k = 0; k < np; k++) death[n - 1 - dob[2 * k + 1]] = (int32_t)(n - 1 - dob[2 * k]);
+  const int64_t np2 = emit_pairs(n, death, pairs_out);
+  free(dcp);
+  free(drw);
+  free(ddm);
+  free(dsk);
+  free(dob);
+  free(death);
+  free(fill);
+  return np2;
+}
+
 /* ------------------------------------------------------------------------------------ */
 int64_t or_dense_standard(int64_t n, const int64_t* colptr, const int32_t* rows,
                           const uint8_t* skip, int64_t* pairs_out) {

@@ -20,6 +20,11 @@
  *                    random tiny complexes; Betti numbers of triangulated spheres / tori /
  *                    projective plane (closed forms); pair counts of the shipped _maxclear
  *                    fixtures equal those of their originals (data/Alpha, byte-identical).
+ *  or_twist_reduce_dual  specseq.h:303-363,1188 (dualize: twist reduction of the
+ *                    anti-transpose, pairs mapped back as (b', d') -> (n-1-d', n-1-b')).
+ *                    Pinned by agreement with or_twist_reduce (persistence duality: homology
+ *                    and cohomology pairs coincide) and with or_rank_pairs on random tiny
+ *                    complexes, closed forms and the shipped fixtures.
  *  or_dense_standard standard reduction (Edelsbrunner-Harer p.153) on dense bit columns.
  *                    Pinned by or_rank_pairs on tiny inputs.
  *  or_rank_pairs     (i,j) is a pair iff r(i,j)-r(i+1,j)-r(i,j-1)+r(i+1,j-1)=1, with r the
@@ -55,6 +60,12 @@ int or_save_pairs(const char* path, int binary, int64_t np, const int64_t* pairs
 int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
                         const int32_t* dims, const uint8_t* skip, int64_t* pairs_out,
                         double* stats_out);
+
+/* The same pairs by twist reduction of the anti-transpose (cohomology), mapped back.  Same
+ * output contract; stats_out counts the dual reduction's additions. */
+int64_t or_twist_reduce_dual(int64_t n, const int64_t* colptr, const int32_t* rows,
+                             const int32_t* dims, const uint8_t* skip, int64_t* pairs_out,
+                             double* stats_out);
 
 /* Standard algorithm on dense bit columns (n <= 65536). Same output contract. */
 int64_t or_dense_standard(int64_t n, const int64_t* colptr, const int32_t* rows,

@@ -112,3 +112,41 @@ def test_planted_mistakes_are_caught():
     # a collapse that keeps every edge differs from GUDHI's kept set on a dense graph
     u, v, kept = F.kept_edge_cases()[1]
     assert not np.array_equal(np.ones_like(kept), kept)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_cohomology_twist_matches_homology_and_rank_formula(seed):
+    """Persistence duality: the twist reduction of the anti-transpose gives the same pairs as
+    the primal twist and as the rank formula (tiny inputs), with and without a sub-complex."""
+    X = C.uniform_points(12 + seed % 6, 3, 300 + seed)
+    m = C.vr_matrix(X, 0.55, 3)
+    if m.n > 128:
+        m = C.vr_matrix(X, 0.4, 2)
+    p = O.twist_pairs(m, dual=True)
+    assert np.array_equal(p, O.twist_pairs(m))
+    if m.n <= 96:
+        assert np.array_equal(p, O.rank_pairs(m))
+    if (m.dims == 1).sum() > 2:
+        skip = O.collapse_skip(m)
+        assert np.array_equal(O.twist_pairs(m, skip=skip, dual=True), O.twist_pairs(m, skip=skip))
+
+
+@pytest.mark.parametrize("name,tri", [("torus", C.TORUS_7), ("rp2", C.RP2_6), ("ball", C.BALL_3)])
+def test_cohomology_twist_closed_forms(name, tri):
+    m = C.closure_matrix(tri, 5)
+    assert np.array_equal(O.twist_pairs(m, dual=True), O.rank_pairs(m))
+
+
+def test_cohomology_twist_on_fixtures():
+    for name in F.DATA_FIXTURES:
+        m = O.load_phat(F.fixture_path(name))
+        assert np.array_equal(O.twist_pairs(m, dual=True), F.expected_pairs(name)), name
+
+
+def test_cohomology_planted_mistake_is_caught():
+    """A dual that maps pairs back without the reversal (b', d') -> (b', d') disagrees."""
+    m = C.closure_matrix(C.TORUS_7, 1)
+    p = O.twist_pairs(m, dual=True)
+    n = m.n
+    wrong = np.stack([n - 1 - p[:, 0], n - 1 - p[:, 1]], 1)
+    assert not np.array_equal(wrong[np.argsort(wrong[:, 0])], O.rank_pairs(m))

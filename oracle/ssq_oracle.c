@@ -48,7 +48,7 @@ static int mb_push_col(mbuild* m, int32_t dim, const int32_t* r, int64_t k) {
     if (!m->rows) return -1;
   }
   if (m->n == 0) m->colptr[0] = 0;
-  memcpy(m->rows + m->nnz, r, sizeof(int32_t) * (size_t)k);
+  if (k > 0) memcpy(m->rows + m->nnz, r, sizeof(int32_t) * (size_t)k);
   int64_t kk = sort_unique_i32(m->rows + m->nnz, k); /* main.cu:118-119 sort + unique */
   m->nnz += kk;
   m->dims[m->n] = dim;

@@ -177,6 +177,21 @@ int ssq_batch_reduce_points(ssq_handle* h, int count, const float* const* pts, c
                             const float* r, int maxdim, const ssq_params* p, ssq_pairs* outs,
                             ssq_stats* stats);
 
+/* ---- Batched Rips cohomology with everything in HBM (no reference counterpart; the batched
+ * stream's hot path).  Clouds of at most 4096 points each, maxdim 2.  d_pts holds every cloud's
+ * points (pbase[count] x pdim float32, device memory); cloud c is points [pbase[c], pbase[c+1])
+ * with threshold radius[c] (pbase, radius: host arrays).  The coboundary matrices are built on
+ * the GPU straight into the engine's dual layout, in groups sized to the free device memory,
+ * and reduced in cohomology (the pairs equal those of ssq_reduce on ssq_vr_build's matrix;
+ * dualize is implied, model bit 1 is not supported).  Pairs, sorted by birth in each cloud's
+ * own indices, go to device memory: cloud c's are d_pairs[2*pair_off[c] .. 2*pair_off[c+1]).
+ * pairs_cap must be at least the number of vertices plus edges (ssq_rips_count). */
+int ssq_rips_count(ssq_handle* h, int count, const float* d_pts, const int64_t* pbase, int pdim,
+                   const float* radius, int64_t* edges, int64_t* triangles);
+int ssq_batch_reduce_points_device(ssq_handle* h, int count, const float* d_pts, const int64_t* pbase, int pdim,
+                                   const float* radius, int maxdim, const ssq_params* p, int64_t* d_pairs,
+                                   int64_t pairs_cap, int64_t* pair_off, ssq_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,12 @@ SIGNATURES = {
                                                ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                                ctypes.POINTER(ssq_params), ctypes.POINTER(ssq_pairs),
                                                ctypes.POINTER(ssq_stats)]),
+    "ssq_rips_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64p, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_float), i64p, i64p]),
+    "ssq_batch_reduce_points_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, i64p,
+                                                      ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                                      ctypes.POINTER(ssq_params), ctypes.c_void_p, ctypes.c_int64,
+                                                      i64p, ctypes.POINTER(ssq_stats)]),
 }
 
 _lib = None

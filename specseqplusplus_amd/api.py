@@ -165,6 +165,34 @@ class Handle:
         res = [_pairs_from_c(outs[i]) for i in range(n)]
         return (res, st.as_dict()) if stats else res
 
+    def rips_count(self, d_points, pbase, radius, pdim=3):
+        """Exact edge and triangle counts of each cloud (points resident: d_points a
+        DeviceArray of float32, cloud c = rows pbase[c]:pbase[c+1])."""
+        pb = np.ascontiguousarray(pbase, dtype=np.int64)
+        rad = np.ascontiguousarray(radius, dtype=np.float32)
+        c = len(rad)
+        e = np.zeros(c, np.int64)
+        t = np.zeros(c, np.int64)
+        L.check(L.lib().ssq_rips_count(self._h, c, d_points.ptr, pb.ctypes.data_as(L.i64p), int(pdim),
+                                       rad.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), e.ctypes.data_as(L.i64p),
+                                       t.ctypes.data_as(L.i64p)), self.error())
+        return e, t
+
+    def batch_reduce_points_device(self, d_points, pbase, radius, d_pairs, pairs_cap, p=None, pdim=3, stats=False):
+        """Batched Rips cohomology with points and pairs in HBM: returns the per-cloud pair
+        offsets (cloud c's pairs are d_pairs[2*off[c] : 2*off[c+1]])."""
+        p = p or params(dualize=1, ss_block_size=1 << 30)
+        pb = np.ascontiguousarray(pbase, dtype=np.int64)
+        rad = np.ascontiguousarray(radius, dtype=np.float32)
+        c = len(rad)
+        off = np.zeros(c + 1, np.int64)
+        st = L.ssq_stats()
+        L.check(L.lib().ssq_batch_reduce_points_device(self._h, c, d_points.ptr, pb.ctypes.data_as(L.i64p), int(pdim),
+                                                       rad.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 2,
+                                                       ctypes.byref(p), d_pairs.ptr, int(pairs_cap),
+                                                       off.ctypes.data_as(L.i64p), ctypes.byref(st)), self.error())
+        return (off, st.as_dict()) if stats else off
+
     def device_array(self, arr):
         """Copy a numpy array into a new device buffer; returns a DeviceArray."""
         arr = np.ascontiguousarray(arr)

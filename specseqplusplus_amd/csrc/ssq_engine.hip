@@ -75,7 +75,8 @@ __device__ __forceinline__ PivMeta pivmeta_of(int64_t off, int32_t len, const in
 }
 
 struct Dev {
-  int64_t n;
+  int64_t n;           // cells (rows)
+  int64_t ncols;       // cells [0, ncols) have columns; the rest are rows only (<= n)
   PivMeta* pivmeta;    // row -> storage of piv[row]
   const int32_t* dims;
   const int64_t* colptr;
@@ -191,8 +192,10 @@ __device__ bool wave_alloc(const Dev& d, int32_t need, int64_t* off, int32_t* ca
 // ------------------------------------------------------------------------------------
 // setup and init passes
 // ------------------------------------------------------------------------------------
-__global__ void k_setup(Dev d) {
-  GRID_STRIDE(c, d.n) {
+// Column state over [0, ncols); with init_rows (ncols == n) also the per-row tables, which a
+// prebuilt matrix brings initialised instead.
+__global__ void k_setup(Dev d, int init_rows) {
+  GRID_STRIDE(c, d.ncols) {
     const int64_t o = d.colptr[c];
     const int32_t l = (int32_t)(d.colptr[c + 1] - o);
     const bool rem = d.removed[c] != 0;
@@ -201,10 +204,12 @@ __global__ void k_setup(Dev d) {
     d.len[c] = rem ? 0 : l;
     d.low[c] = (rem || l == 0) ? -1 : d.pool[o + l - 1];
     d.state[c] = (rem || l == 0) ? ST_ZERO : ST_PENDING;
-    d.piv[c] = kEmpty;
-    d.leftone[c] = kEmpty;
-    d.flag[c] = 0;
-    d.flag[d.n + c] = 0;
+    if (init_rows) {
+      d.piv[c] = kEmpty;
+      d.leftone[c] = kEmpty;
+      d.flag[c] = 0;
+      d.flag[d.n + c] = 0;
+    }
   }
 }
 
@@ -225,7 +230,7 @@ __global__ void k_leftone(Dev d) {
 __global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
   uint32_t apparent = 0;  // summed per wavefront: one atomic per wave, not per pair
   uint32_t negdims = 0;
-  GRID_STRIDE(c, d.n) {
+  GRID_STRIDE(c, d.ncols) {
     const int32_t l = d.len[c];
     if (l == 0) continue;
     const int32_t lw = d.low[c];
@@ -260,7 +265,7 @@ __global__ void k_lookup(Dev d, const uint8_t* hdclear, int neg_known) {
   if (lane_id() == 0 && apparent) atomicAdd(&d.ctr[cw::kApparent], apparent);
   if (lane_id() == 0 && negdims) atomicOr(&d.ctr[cw::kNegDims], negdims);
   if (hdclear)
-    GRID_STRIDE(c, d.n) if (hdclear[c]) d.flag[c] = 1;
+    GRID_STRIDE(c, d.ncols) if (hdclear[c]) d.flag[c] = 1;
 }
 
 
@@ -303,7 +308,7 @@ __global__ void k_clear_compress_stats(Dev d, int32_t* maxrow, int32_t* minrow) 
     mx[k] = -1;
     mn[k] = 0x7fffffff;
   }
-  GRID_STRIDE(c, d.n) {
+  GRID_STRIDE(c, d.ncols) {
     uint8_t st = d.state[c];
     if (st == ST_ZERO) continue;
     if (d.flag[c] && st != ST_PIVOT) {  // clearing (kernel.h:483-493)
@@ -1026,13 +1031,14 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
   }
 }
 
-// Twist clearing at the end of a dimension (specseq.h:1039-1046).
+// Twist clearing at the end of a dimension (specseq.h:1039-1046).  A low row without a column
+// (cells >= ncols of a prebuilt matrix) has nothing to clear.
 __global__ void k_twist(Dev d, const int32_t* cols, int32_t ncols) {
   GRID_STRIDE(i, ncols) {
     const int32_t c = cols[i];
     if (d.state[c] == ST_PIVOT) {
       const int32_t l = d.low[c];
-      if (d.state[l] == ST_PENDING) d.state[l] = ST_ZERO;
+      if (l < d.ncols && d.state[l] == ST_PENDING) d.state[l] = ST_ZERO;
     }
   }
 }
@@ -1666,10 +1672,10 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
-class Engine {
+class Engine : public PrebuiltTarget {
  public:
   explicit Engine(int device) : device_(device) {}
-  ~Engine() { release(); }
+  ~Engine() override { release(); }
 
   int init() {
     int ndev = 0;
@@ -1716,7 +1722,42 @@ class Engine {
                     const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
                     ssq_stats* st, int64_t* d_essential = nullptr, int64_t* ne_out = nullptr);
 
+  // Prebuilt matrices: a builder writes the rows and the init tables straight into the engine's
+  // buffers, then runs the init passes and the page loop on them.  Cells [0, n) are rows;
+  // cells [0, ncols) also have columns (colptr[ncols + 1] into the pool, dims[ncols]; both stay
+  // the caller's).  prebuilt_begin sizes the buffers and returns:
+  //   pool[0, nnz): the rows to write; pool[nnz, nnz + arena): free (the store's arena, usable as
+  //     scratch until prebuilt_reduce);
+  //   leftone[n]: every row's leftmost column (kEmpty where none), to write;
+  //   negflag[n]: zeroed; set to 1 for every column that is leftmost in some row.
+  struct PrebuiltBufs {
+    int32_t* pool;
+    uint32_t* leftone;
+    uint8_t* negflag;
+  };
+  int prebuilt_begin(int64_t n, int64_t ncols, int64_t nnz, int64_t arena, const int64_t* colptr, const int32_t* dims,
+                     int32_t maxdim, PrebuiltBufs* out);
+  // Reduces the prebuilt matrix.  *oom: the arena ran out (rebuild with a larger one).  After
+  // it, low/state per column and the row -> column pivot table describe the pairs.
+  int prebuilt_reduce(const ssq_params& p, ssq_stats* st, bool* oom);
+  // PrebuiltTarget
+  int pre_begin(int64_t n, int64_t ncols, int64_t nnz, int64_t arena, const int64_t* colptr, const int32_t* dims,
+                int32_t maxdim, int32_t** pool, uint32_t** leftone, uint8_t** negflag) override {
+    PrebuiltBufs b{};
+    const int rc = prebuilt_begin(n, ncols, nnz, arena, colptr, dims, maxdim, &b);
+    *pool = b.pool;
+    *leftone = b.leftone;
+    *negflag = b.negflag;
+    return rc;
+  }
+  int pre_reduce(const ssq_params& p, ssq_stats* st, bool* oom) override { return prebuilt_reduce(p, st, oom); }
+  const int32_t* pre_low() const override { return pre_.low; }
+  const uint32_t* pre_piv() const override { return pre_.piv; }
+  hipStream_t pre_stream() const override { return stream_; }
+  const std::string& pre_error() const override { return err_; }
+
  private:
+  void fill_stats(ssq_stats* S, int64_t nnz, int64_t arena_cap);
   void set_error(const std::string& s) { err_ = s; }
   template <typename T>
   int ensure(DevBuf& b, size_t count, T** out) {
@@ -1777,6 +1818,16 @@ class Engine {
   int guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dims, const int64_t* d_colptr,
                    const int32_t* d_rows, const ssq_params& p, uint8_t** hd_out, ssq_stats* st);
   int scan_counts(const int32_t* cnt, int64_t n, int64_t* out);
+  struct LoopPlan {
+    int32_t* U = nullptr;  // pending columns, grouped by dimension
+    int64_t cnt[kMaxDims] = {}, ubase[kMaxDims] = {}, upend = 0;
+    int32_t maxrow[kMaxDims] = {}, minrow[kMaxDims] = {};
+  };
+  int init_lists(Dev& d, int32_t maxdim, const uint8_t* hd, bool neg_known, const ssq_params& p, LoopPlan* L,
+                 ssq_stats* S);
+  int page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPlan& L, ssq_stats* S, bool* oom_out);
+  Dev pre_{};                // the prebuilt matrix between prebuilt_begin and prebuilt_reduce
+  int32_t pre_maxdim_ = 0;
   int build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
                  const int32_t* dims, bool any_removed, int64_t** dcolptr, int32_t** drows, int32_t** ddims);
   int dual_by_sort(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, const int32_t* rows, bool any_removed,
@@ -2145,6 +2196,146 @@ int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int
   return SSQ_OK;
 }
 
+// Init passes on a matrix whose rows are in the column store and whose column state k_setup
+// has written (leftone set; with neg_known also the negative flags): apparent pairs,
+// clearing, compression, then the per-dimension pending lists, page row bounds and level
+// slots, read back in one round trip.
+int Engine::init_lists(Dev& d, int32_t maxdim, const uint8_t* hd, bool neg_known, const ssq_params& p, LoopPlan* L,
+                       ssq_stats* S) {
+  const int g = grid_for(d.ncols);
+  hipLaunchKernelGGL(k_lookup, dim3(g), dim3(256), 0, stream_, d, hd, neg_known ? 1 : 0);
+  int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
+  hipLaunchKernelGGL(k_rowstat_init, dim3(1), dim3(64), 0, stream_, rowstat);
+  hipLaunchKernelGGL(k_clear_compress_stats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
+  HIPCHK(hipGetLastError());
+  // pending lists per dimension and page bounds, read back in one round trip
+  const int pend_keys = std::min(kMaxDims, std::max(maxdim, 1) + 1);  // dims 1..maxdim hold pending columns
+  HIPCHK(hipMemsetAsync(tot_ + kTotPending, 0, sizeof(int64_t) * kMaxDims, stream_));
+  int rc = compact(d.ncols, pend_keys, KeyPendingDim{d.state, d.dims, pend_keys}, EmitIndex{L->U}, nullptr, kTotPending);
+  if (rc) return rc;
+
+  if ((rc = sync_all())) return rc;
+  int64_t upend = 0;
+  for (int k = 0; k < kMaxDims; k++) {
+    L->cnt[k] = host_tot_[kTotPending + k];
+    L->ubase[k] = upend;
+    upend += L->cnt[k];
+  }
+  for (int k = 0; k < kMaxDims; k++) {
+    L->maxrow[k] = (int32_t)host_ctr_[cw::kRowStat + k];
+    L->minrow[k] = (int32_t)host_ctr_[cw::kRowStat + kMaxDims + k];
+  }
+  S->n_apparent = host_ctr_[cw::kApparent];
+  S->n_pending = upend;
+  L->upend = upend;
+  // level slots (specseq.h:281-283), arena-backed
+  d.lvl_cap[1] = p.level1;
+  d.lvl_cap[2] = p.level2;
+  d.lvl_cap[3] = (int32_t)std::min<int64_t>((int64_t)p.level2 * 8, 0x3fffffff);
+  const int64_t U1 = upend;
+  d.lvl_slots[1] = (uint32_t)std::min<int64_t>(U1 + 1, 0xffffffffLL);
+  d.lvl_slots[2] = (uint32_t)std::min<int64_t>(U1 / std::max(1, p.level2 / p.level1) / 2 + 1, 0xffffffffLL);
+  d.lvl_slots[3] = (uint32_t)std::min<int64_t>(U1 / std::max<int64_t>(1, (int64_t)d.lvl_cap[3] / p.level1) / 2 + 1,
+                                               0xffffffffLL);
+  return SSQ_OK;
+}
+
+// Claim/add rounds per dimension, high to low (specseq.h:598-704), each page until its
+// worklist is empty, then twist clearing.  *oom_out: the column store ran out (the caller
+// reruns with a larger arena).
+int Engine::page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPlan& L, ssq_stats* S, bool* oom_out) {
+  // page loop, dimensions high to low (specseq.h:598-704)
+  const int64_t upend = L.upend;
+  int32_t* W0;
+  int rc = ensure(b_work_, (size_t)(5 * std::max<int64_t>(upend, 1) + 64), &W0);
+  if (rc) return rc;
+  const int64_t wcap = std::max<int64_t>(upend, 1);
+  for (int k = 0; k < 5; k++) d.work[k] = W0 + k * wcap;
+  const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * add_waves_per_cu_);
+  const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
+  const int rounds_per_sync = rounds_per_sync_;
+  const int long_grid = cus_ * 2;
+  bool oom = false;
+  *oom_out = false;
+  for (int32_t dim = maxdim; dim >= 1 && !oom; dim--) {
+    const int64_t nl = L.cnt[dim];
+    if (nl == 0 || L.maxrow[dim] < 0) continue;
+    const int32_t* cols = L.U + L.ubase[dim];
+    int32_t ss = p.ss_block_size;
+    int64_t begin = 0, end = std::min<int64_t>(ss, nl);
+    int64_t mrow = (int64_t)L.maxrow[dim] - ss, xrow = mrow + ss;
+    const int64_t dminrow = L.minrow[dim];
+    while ((begin < nl || xrow > dminrow) && !oom) {
+      HIPCHK(hipMemsetAsync(d.ctr, 0, 8 * sizeof(uint32_t), stream_));
+      const int64_t wn = end - begin;
+      if (wn > 0) {
+        const int32_t fl = (int32_t)std::max<int64_t>(mrow, INT32_MIN);
+        hipLaunchKernelGGL(k_scan_window, dim3(grid_for(wn)), dim3(256), 0, stream_, d, cols + begin, (int32_t)wn,
+                           fl);
+        const int cg = grid_for(wn);
+        int r = 0;
+        for (;;) {
+          for (int t = 0; t < rounds_per_sync; t++, r++) {
+            hipLaunchKernelGGL(k_claim, dim3(cg), dim3(256), 0, stream_, d, r);
+            hipLaunchKernelGGL(k_pivmeta, dim3(cg), dim3(256), 0, stream_, d, r);
+            if (timing_) HIPCHK(hipEventRecord(ev_[4 * t], stream_));
+            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(kWave * kAddWaves), 0, stream_, d, r, fl);
+            if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 1], stream_));
+            hipLaunchKernelGGL(k_add_long, dim3(long_grid), dim3(kLongThreads), 0, stream_, d, r, fl);
+            if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 2], stream_));
+          }
+          HIPCHK(hipGetLastError());
+          if ((rc = sync_read(d.ctr, cw::kWords))) return rc;
+          S->n_add_launches += rounds_per_sync;
+          if (timing_)
+            for (int t = 0; t < rounds_per_sync; t++) {
+              float ms = 0.f, ms2 = 0.f;
+              HIPCHK(hipEventElapsedTime(&ms, ev_[4 * t], ev_[4 * t + 1]));
+              HIPCHK(hipEventElapsedTime(&ms2, ev_[4 * t + 1], ev_[4 * t + 2]));
+              S->t_add_kernel_ms += ms;
+              S->t_add_long_kernel_ms += ms2;
+              if (trace_)
+                std::fprintf(stderr, "[ssq] dim %d page %lld round %d: k_add %.3f ms, k_add_long %.3f ms\n", dim,
+                             (long long)S->n_pages, r - rounds_per_sync + t, ms, ms2);
+            }
+          if (trace_)
+            std::fprintf(stderr,
+                         "[ssq] dim %d page %lld after round %d: window %lld, next worklist %u, last long list %u, "
+                         "additions so far %llu, longest chain %u, longest |a|+|b| %u\n",
+                         dim, (long long)S->n_pages, r - 1, (long long)wn, host_ctr_[cw::kW + (r & 1)],
+                         host_ctr_[cw::kLong], ((const unsigned long long*)(host_ctr_ + cw::kStats))[0], host_ctr_[cw::kChain],
+                         host_ctr_[cw::kChain + 1]);
+          S->n_rounds += rounds_per_sync;
+          if (host_ctr_[cw::kErr]) {
+            oom = true;
+            break;
+          }
+          if (host_ctr_[cw::kW + (r & 1)] == 0) break;
+          if (r > (1 << 24)) {
+            err_ = "page did not converge";
+            return SSQ_EDEVICE;
+          }
+        }
+      }
+      S->n_pages++;
+      // a page that covered every column with every row active finished the dimension
+      if (end >= nl && mrow <= dminrow) break;
+      ss = (int32_t)((float)ss * p.growth);
+      if (ss < 1) ss = 1;
+      if (xrow <= dminrow) begin = end;
+      else {
+        mrow -= ss;
+        xrow -= ss;
+      }
+      if (end < nl) end = std::min<int64_t>(end + ss, nl);
+    }
+    if (oom) break;
+    hipLaunchKernelGGL(k_twist, dim3(grid_for(nl)), dim3(256), 0, stream_, d, cols, (int32_t)nl);
+  }
+  *oom_out = oom;
+  return SSQ_OK;
+}
+
 int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
                           const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
                           ssq_stats* st, int64_t* d_essential, int64_t* ne_out) {
@@ -2167,6 +2358,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
 
   Dev d{};
   d.n = n;
+  d.ncols = n;
   d.short_max = short_max_;
   d.long_window = long_window_;
   d.dims = d_dims;
@@ -2256,138 +2448,20 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
                           stream_));
     HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
     const int g = grid_for(n);
-    hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d);
+    hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d, 1);
     if (dual)
       hipLaunchKernelGGL(k_dual_init_from_primal, dim3(g), dim3(256), 0, stream_, n, primal_colptr, primal_rows,
                          (const uint8_t*)primal_removed, d.leftone, d.flag + n);
     else
       hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
-    hipLaunchKernelGGL(k_lookup, dim3(g), dim3(256), 0, stream_, d, (const uint8_t*)hd, dual ? 1 : 0);
-    int32_t* rowstat = (int32_t*)(d.ctr + cw::kRowStat);
-    hipLaunchKernelGGL(k_rowstat_init, dim3(1), dim3(64), 0, stream_, rowstat);
-    hipLaunchKernelGGL(k_clear_compress_stats, dim3(g), dim3(256), 0, stream_, d, rowstat, rowstat + kMaxDims);
-    HIPCHK(hipGetLastError());
-    // pending lists per dimension and page bounds, read back in one round trip
-    const int pend_keys = std::min(kMaxDims, std::max(maxdim, 1) + 1);  // dims 1..maxdim hold pending columns
-    HIPCHK(hipMemsetAsync(tot_ + kTotPending, 0, sizeof(int64_t) * kMaxDims, stream_));
-    rc = compact(n, pend_keys, KeyPendingDim{d.state, d.dims, pend_keys}, EmitIndex{U}, nullptr, kTotPending);
-    if (rc) return rc;
-
-    rc = sync_all();
-    if (rc) return rc;
-    int64_t cnt[kMaxDims];
-    int64_t ubase[kMaxDims];
-    int64_t upend = 0;
-    for (int k = 0; k < kMaxDims; k++) {
-      cnt[k] = host_tot_[kTotPending + k];
-      ubase[k] = upend;
-      upend += cnt[k];
-    }
-    int32_t maxrow[kMaxDims], minrow[kMaxDims];
-    for (int k = 0; k < kMaxDims; k++) {
-      maxrow[k] = (int32_t)host_ctr_[cw::kRowStat + k];
-      minrow[k] = (int32_t)host_ctr_[cw::kRowStat + kMaxDims + k];
-    }
-    S->n_apparent = host_ctr_[cw::kApparent];
-    S->n_pending = upend;
-    // level slots (specseq.h:281-283), arena-backed
-    d.lvl_cap[1] = p.level1;
-    d.lvl_cap[2] = p.level2;
-    d.lvl_cap[3] = (int32_t)std::min<int64_t>((int64_t)p.level2 * 8, 0x3fffffff);
-    const int64_t U1 = upend;
-    d.lvl_slots[1] = (uint32_t)std::min<int64_t>(U1 + 1, 0xffffffffLL);
-    d.lvl_slots[2] = (uint32_t)std::min<int64_t>(U1 / std::max(1, p.level2 / p.level1) / 2 + 1, 0xffffffffLL);
-    d.lvl_slots[3] = (uint32_t)std::min<int64_t>(U1 / std::max<int64_t>(1, (int64_t)d.lvl_cap[3] / p.level1) / 2 + 1,
-                                                 0xffffffffLL);
+    LoopPlan L;
+    L.U = U;
+    if ((rc = init_lists(d, maxdim, (const uint8_t*)hd, dual, p, &L, S))) return rc;
     auto T1 = std::chrono::steady_clock::now();
     S->t_init_ms = std::chrono::duration<double, std::milli>(T1 - T0).count();
 
-    // page loop, dimensions high to low (specseq.h:598-704)
-    int32_t* W0;
-    if ((rc = ensure(b_work_, (size_t)(5 * std::max<int64_t>(upend, 1) + 64), &W0))) return rc;
-    const int64_t wcap = std::max<int64_t>(upend, 1);
-    for (int k = 0; k < 5; k++) d.work[k] = W0 + k * wcap;
-    const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * add_waves_per_cu_);
-    const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
-    const int rounds_per_sync = rounds_per_sync_;
-    const int long_grid = cus_ * 2;
     bool oom = false;
-    for (int32_t dim = maxdim; dim >= 1 && !oom; dim--) {
-      const int64_t L = cnt[dim];
-      if (L == 0 || maxrow[dim] < 0) continue;
-      const int32_t* cols = U + ubase[dim];
-      int32_t ss = p.ss_block_size;
-      int64_t begin = 0, end = std::min<int64_t>(ss, L);
-      int64_t mrow = (int64_t)maxrow[dim] - ss, xrow = mrow + ss;
-      const int64_t dminrow = minrow[dim];
-      while ((begin < L || xrow > dminrow) && !oom) {
-        HIPCHK(hipMemsetAsync(d.ctr, 0, 8 * sizeof(uint32_t), stream_));
-        const int64_t wn = end - begin;
-        if (wn > 0) {
-          const int32_t fl = (int32_t)std::max<int64_t>(mrow, INT32_MIN);
-          hipLaunchKernelGGL(k_scan_window, dim3(grid_for(wn)), dim3(256), 0, stream_, d, cols + begin, (int32_t)wn,
-                             fl);
-          const int cg = grid_for(wn);
-          int r = 0;
-          for (;;) {
-            for (int t = 0; t < rounds_per_sync; t++, r++) {
-              hipLaunchKernelGGL(k_claim, dim3(cg), dim3(256), 0, stream_, d, r);
-              hipLaunchKernelGGL(k_pivmeta, dim3(cg), dim3(256), 0, stream_, d, r);
-              if (timing_) HIPCHK(hipEventRecord(ev_[4 * t], stream_));
-              hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(kWave * kAddWaves), 0, stream_, d, r, fl);
-              if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 1], stream_));
-              hipLaunchKernelGGL(k_add_long, dim3(long_grid), dim3(kLongThreads), 0, stream_, d, r, fl);
-              if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 2], stream_));
-            }
-            HIPCHK(hipGetLastError());
-            rc = sync_read(d.ctr, cw::kWords);
-            if (rc) return rc;
-            S->n_add_launches += rounds_per_sync;
-            if (timing_)
-              for (int t = 0; t < rounds_per_sync; t++) {
-                float ms = 0.f, ms2 = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, ev_[4 * t], ev_[4 * t + 1]));
-                HIPCHK(hipEventElapsedTime(&ms2, ev_[4 * t + 1], ev_[4 * t + 2]));
-                S->t_add_kernel_ms += ms;
-                S->t_add_long_kernel_ms += ms2;
-                if (trace_)
-                  std::fprintf(stderr, "[ssq] dim %d page %lld round %d: k_add %.3f ms, k_add_long %.3f ms\n", dim,
-                               (long long)S->n_pages, r - rounds_per_sync + t, ms, ms2);
-              }
-            if (trace_)
-              std::fprintf(stderr,
-                           "[ssq] dim %d page %lld after round %d: window %lld, next worklist %u, last long list %u, "
-                           "additions so far %llu, longest chain %u, longest |a|+|b| %u\n",
-                           dim, (long long)S->n_pages, r - 1, (long long)wn, host_ctr_[cw::kW + (r & 1)],
-                           host_ctr_[cw::kLong], ((const unsigned long long*)(host_ctr_ + cw::kStats))[0], host_ctr_[cw::kChain],
-                           host_ctr_[cw::kChain + 1]);
-            S->n_rounds += rounds_per_sync;
-            if (host_ctr_[cw::kErr]) {
-              oom = true;
-              break;
-            }
-            if (host_ctr_[cw::kW + (r & 1)] == 0) break;
-            if (r > (1 << 24)) {
-              err_ = "page did not converge";
-              return SSQ_EDEVICE;
-            }
-          }
-        }
-        S->n_pages++;
-        // a page that covered every column with every row active finished the dimension
-        if (end >= L && mrow <= dminrow) break;
-        ss = (int32_t)((float)ss * p.growth);
-        if (ss < 1) ss = 1;
-        if (xrow <= dminrow) begin = end;
-        else {
-          mrow -= ss;
-          xrow -= ss;
-        }
-        if (end < L) end = std::min<int64_t>(end + ss, L);
-      }
-      if (oom) break;
-      hipLaunchKernelGGL(k_twist, dim3(grid_for(L)), dim3(256), 0, stream_, d, cols, (int32_t)L);
-    }
+    if ((rc = page_loop(d, p, maxdim, L, S, &oom))) return rc;
     if (oom) {
       const int64_t budget = (int64_t)1 << 36;  // 256 GiB of int32 entries at most
       if (arena_cap * 2 > budget || attempt > 8) {
@@ -2422,21 +2496,99 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     if (rc) return rc;
     *np_out = host_tot_[kTotPairs];
     if (d_essential && ne_out) *ne_out = host_tot_[kTotEss];
-    const unsigned long long* s64 = (const unsigned long long*)(host_ctr_ + cw::kStats);
-    S->n_additions = (int64_t)s64[0];
-    S->add_bytes = (double)s64[1];
-    S->store_bytes = (int64_t)(nnz_eff + arena_cap) * 4;
-    arena_hint_ = std::max(arena_hint_, (int64_t)(*(const unsigned long long*)(host_ctr_ + cw::kArena)) * 2);
-    S->n_tiled_additions = (int64_t)s64[2];
-    S->n_dense_additions = (int64_t)s64[3];
-    S->dense_bytes = (double)s64[4];
-    S->max_chain = host_ctr_[cw::kChain];
-    S->max_addition_len = host_ctr_[cw::kChain + 1];
+    fill_stats(S, nnz_eff, arena_cap);
     auto T2 = std::chrono::steady_clock::now();
     S->t_reduce_ms = std::chrono::duration<double, std::milli>(T2 - T1).count();
     S->t_total_ms = std::chrono::duration<double, std::milli>(T2 - T0).count();
     return SSQ_OK;
   }
+}
+
+// Counters of the last page loop (host_ctr_ must hold the device counters).
+void Engine::fill_stats(ssq_stats* S, int64_t nnz, int64_t arena_cap) {
+  const unsigned long long* s64 = (const unsigned long long*)(host_ctr_ + cw::kStats);
+  S->n_additions = (int64_t)s64[0];
+  S->add_bytes = (double)s64[1];
+  S->store_bytes = (nnz + arena_cap) * 4;
+  arena_hint_ = std::max(arena_hint_, (int64_t)(*(const unsigned long long*)(host_ctr_ + cw::kArena)) * 2);
+  S->n_tiled_additions = (int64_t)s64[2];
+  S->n_dense_additions = (int64_t)s64[3];
+  S->dense_bytes = (double)s64[4];
+  S->max_chain = host_ctr_[cw::kChain];
+  S->max_addition_len = host_ctr_[cw::kChain + 1];
+}
+
+int Engine::prebuilt_begin(int64_t n, int64_t ncols, int64_t nnz, int64_t arena, const int64_t* colptr,
+                           const int32_t* dims, int32_t maxdim, PrebuiltBufs* out) {
+  err_.clear();
+  if (n < 1 || ncols < 1 || ncols > n || n >= 0x7fffffffLL || nnz < 0 || arena < 0 || !colptr || !dims || !out ||
+      maxdim < 1 || maxdim >= kMaxDims) {
+    err_ = "invalid prebuilt matrix";
+    return SSQ_EINVAL;
+  }
+  HIPCHK(hipSetDevice(device_));
+  Dev d{};
+  d.n = n;
+  d.ncols = ncols;
+  d.short_max = short_max_;
+  d.long_window = long_window_;
+  d.dims = dims;
+  d.colptr = colptr;
+  int rc;
+  if ((rc = ensure(b_off_, (size_t)ncols, &d.off)) || (rc = ensure(b_len_, (size_t)ncols, &d.len)) ||
+      (rc = ensure(b_cap_, (size_t)ncols, &d.cap)) || (rc = ensure(b_low_, (size_t)ncols, &d.low)) ||
+      (rc = ensure(b_state_, (size_t)ncols, &d.state)) || (rc = ensure(b_removed_, (size_t)ncols, &d.removed)) ||
+      (rc = ensure(b_flag_, (size_t)(2 * n), &d.flag)) || (rc = ensure(b_piv_, (size_t)n, &d.piv)) ||
+      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)cw::kWords, &d.ctr)) ||
+      (rc = ensure(b_pmeta_, (size_t)n, &d.pivmeta)) || (rc = ensure(b_pool_, (size_t)(nnz + arena), &d.pool)))
+    return rc;
+  d.arena_base = nnz;
+  d.arena_cap = arena;
+  HIPCHK(hipMemsetAsync(d.removed, 0, (size_t)ncols, stream_));
+  HIPCHK(hipMemsetAsync(d.flag, 0, (size_t)(2 * n), stream_));
+  pre_ = d;
+  pre_maxdim_ = maxdim;
+  out->pool = d.pool;
+  out->leftone = d.leftone;
+  out->negflag = d.flag + n;
+  return SSQ_OK;
+}
+
+int Engine::prebuilt_reduce(const ssq_params& p, ssq_stats* st, bool* oom) {
+  *oom = false;
+  if (p.ss_block_size < 1 || !(p.growth >= 1.0f) || p.queue_size < 1 || p.level1 < 1 || p.level2 < 1) {
+    err_ = "invalid parameters";
+    return SSQ_EINVAL;
+  }
+  HIPCHK(hipSetDevice(device_));
+  auto T0 = std::chrono::steady_clock::now();
+  ssq_stats local{};
+  ssq_stats* S = st ? st : &local;
+  std::memset(S, 0, sizeof(*S));
+  Dev& d = pre_;
+  int32_t* U;
+  int rc;
+  if ((rc = ensure(b_U_, (size_t)d.ncols, &U))) return rc;
+  HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
+  ctr_dev_ = d.ctr;
+  d.stats = (unsigned long long*)(d.ctr + cw::kStats);
+  d.arena_used = (unsigned long long*)(d.ctr + cw::kArena);
+  d.lvl_cnt = d.ctr + cw::kLvl;
+  d.err = d.ctr + cw::kErr;
+  HIPCHK(hipMemsetAsync(d.piv, 0xff, sizeof(uint32_t) * d.n, stream_));
+  hipLaunchKernelGGL(k_setup, dim3(grid_for(d.ncols)), dim3(256), 0, stream_, d, 0);
+  LoopPlan L;
+  L.U = U;
+  if ((rc = init_lists(d, pre_maxdim_, nullptr, true, p, &L, S))) return rc;
+  auto T1 = std::chrono::steady_clock::now();
+  S->t_init_ms = std::chrono::duration<double, std::milli>(T1 - T0).count();
+  if ((rc = page_loop(d, p, pre_maxdim_, L, S, oom))) return rc;
+  if ((rc = sync_all())) return rc;
+  fill_stats(S, d.arena_base, d.arena_cap);
+  auto T2 = std::chrono::steady_clock::now();
+  S->t_reduce_ms = std::chrono::duration<double, std::milli>(T2 - T1).count();
+  S->t_total_ms = std::chrono::duration<double, std::milli>(T2 - T0).count();
+  return SSQ_OK;
 }
 
 // ====================================================================================
@@ -2453,9 +2605,13 @@ struct Handle {
   DevBuf in_colptr, in_rows, in_dims, out_pairs, out_ess;
   PinBuf pin_in, pin_out;
   VrBuilder* vr = nullptr;
+  RipsCo* rc = nullptr;
+  DevBuf pts;
   explicit Handle(int dev) : eng(dev) {}
   ~Handle() {
     if (vr) vr_builder_destroy(vr);
+    if (rc) ripsco_destroy(rc);
+    if (pts.p) (void)hipFree(pts.p);
     for (DevBuf* b : {&in_colptr, &in_rows, &in_dims, &out_pairs, &out_ess})
       if (b->p) (void)hipFree(b->p);
     for (PinBuf* b : {&pin_in, &pin_out})
@@ -2788,6 +2944,74 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
   return SSQ_OK;
 }
 
+// Point clouds through the batched Rips coboundary builder: the points go to the device,
+// the pairs come back per cloud, and the essential cells are the complement of the pairs in
+// each cloud's n + edges + triangles cells.
+static int points_cohomology(Handle* H, int count, const float* const* pts, const int64_t* npts, int pdim,
+                             const float* r, const ssq_params& p, ssq_pairs* outs, ssq_stats* stats) {
+  std::vector<int64_t> pbase((size_t)count + 1, 0);
+  for (int i = 0; i < count; i++) {
+    if (npts[i] < 0 || (npts[i] && !pts[i]) || !(r[i] >= 0.f)) return SSQ_EINVAL;
+    pbase[i + 1] = pbase[i] + npts[i];
+  }
+  const int64_t np_all = pbase[count];
+  int rc;
+  if ((rc = pin_grow(H, H->pin_in, sizeof(float) * std::max<int64_t>(np_all * pdim, 1)))) return rc;
+  float* stg = (float*)H->pin_in.p;
+  for (int i = 0; i < count; i++)
+    if (npts[i]) std::memcpy(stg + pbase[i] * pdim, pts[i], sizeof(float) * npts[i] * pdim);
+  if ((rc = dev_grow(H, H->pts, sizeof(float) * std::max<int64_t>(np_all * pdim, 1)))) return rc;
+  hipStream_t s = H->eng.stream();
+  if (np_all && hipMemcpyAsync(H->pts.p, stg, sizeof(float) * np_all * pdim, hipMemcpyHostToDevice, s) != hipSuccess) {
+    H->err = "host to device copy failed";
+    return SSQ_EDEVICE;
+  }
+  if (!H->rc && !(H->rc = ripsco_create())) return SSQ_ENOMEM;
+  std::vector<int64_t> off((size_t)count + 1, 0);
+  const int64_t* dp = nullptr;
+  rc = ripsco_reduce(H->rc, &H->eng, count, (const float*)H->pts.p, pbase.data(), pdim, r, p, nullptr, 0, off.data(),
+                     0, stats, &H->err, &dp);
+  if (rc) return rc;
+  const int64_t np = off[count];
+  if ((rc = pin_grow(H, H->pin_out, sizeof(int64_t) * std::max<int64_t>(2 * np, 1)))) return rc;
+  int64_t* hp = (int64_t*)H->pin_out.p;
+  if ((np && hipMemcpyAsync(hp, dp, sizeof(int64_t) * 2 * np, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    H->err = "device to host copy failed";
+    return SSQ_EDEVICE;
+  }
+  const int64_t* tris = ripsco_triangles(H->rc);
+  const int64_t* edges = ripsco_edges(H->rc);
+  for (int i = 0; i < count; i++) {
+    ssq_pairs* o = &outs[i];
+    const int64_t cells = npts[i] + edges[i] + tris[i];
+    o->np = off[i + 1] - off[i];
+    o->ne = cells - 2 * o->np;
+    o->pairs = (int64_t*)std::malloc(sizeof(int64_t) * std::max<int64_t>(2 * o->np, 1));
+    o->essential = (int64_t*)std::malloc(sizeof(int64_t) * std::max<int64_t>(o->ne, 1));
+    std::vector<uint8_t> paired((size_t)cells, 0);
+    if (!o->pairs || !o->essential || o->ne < 0) {
+      for (int k = 0; k <= i; k++) ssq_pairs_free(&outs[k]);
+      H->err = o->ne < 0 ? "pair count exceeds the cell count" : "host allocation failed";
+      return o->ne < 0 ? SSQ_EDEVICE : SSQ_ENOMEM;
+    }
+    std::memcpy(o->pairs, hp + 2 * off[i], sizeof(int64_t) * 2 * o->np);
+    for (int64_t k = 0; k < 2 * o->np; k++) {
+      const int64_t c = o->pairs[k];
+      if (c >= 0 && c < cells) paired[(size_t)c] = 1;
+    }
+    int64_t e = 0;
+    for (int64_t c = 0; c < cells && e < o->ne; c++)
+      if (!paired[(size_t)c]) o->essential[e++] = c;
+    if (e != o->ne) {
+      for (int k = 0; k <= i; k++) ssq_pairs_free(&outs[k]);
+      H->err = "pairs repeat a cell";
+      return SSQ_EDEVICE;
+    }
+  }
+  return SSQ_OK;
+}
+
 extern "C" int ssq_batch_reduce_points(ssq_handle* h, int count, const float* const* pts, const int64_t* npts,
                                        int pdim, const float* r, int maxdim, const ssq_params* p, ssq_pairs* outs,
                                        ssq_stats* stats) {
@@ -2796,8 +3020,13 @@ extern "C" int ssq_batch_reduce_points(ssq_handle* h, int count, const float* co
   for (int i = 0; i < count; i++) std::memset(&outs[i], 0, sizeof(ssq_pairs));
   Handle* H = h->h;
   H->err.clear();
-  if (!H->vr && !(H->vr = vr_builder_create())) return SSQ_ENOMEM;
   if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  if (maxdim == 2 && p->dualize == 1 && !(p->model & 2) && !std::getenv("SSQ_RIPS_GENERAL")) {
+    bool small = true;
+    for (int i = 0; i < count && small; i++) small = npts[i] >= 0 && npts[i] <= kRipsMaxPoints;
+    if (small) return points_cohomology(H, count, pts, npts, pdim, r, *p, outs, stats);
+  }
+  if (!H->vr && !(H->vr = vr_builder_create())) return SSQ_ENOMEM;
   std::vector<int64_t> base((size_t)count + 1);
   const int64_t* dc;
   const int32_t *dr, *dd;
@@ -2844,4 +3073,31 @@ extern "C" int ssq_vr_build_gpu(ssq_handle* h, const float* pts, int64_t npts, i
       hipStreamSynchronize(s) != hipSuccess)
     return SSQ_EDEVICE;
   return matrix_from_vectors(cp, rw, dm, out);
+}
+
+extern "C" int ssq_rips_count(ssq_handle* h, int count, const float* d_pts, const int64_t* pbase, int pdim,
+                              const float* radius, int64_t* edges, int64_t* triangles) {
+  if (!h || count < 1 || !pbase || !radius || pdim < 1) return SSQ_EINVAL;
+  Handle* H = h->h;
+  H->err.clear();
+  if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  if (!H->rc && !(H->rc = ripsco_create())) return SSQ_ENOMEM;
+  const int rc = ripsco_count(H->rc, H->eng.stream(), count, d_pts, pbase, pdim, radius, edges, triangles);
+  if (rc) H->err = "Rips count failed";
+  return rc;
+}
+
+extern "C" int ssq_batch_reduce_points_device(ssq_handle* h, int count, const float* d_pts, const int64_t* pbase,
+                                              int pdim, const float* radius, int maxdim, const ssq_params* p,
+                                              int64_t* d_pairs, int64_t pairs_cap, int64_t* pair_off,
+                                              ssq_stats* stats) {
+  if (!h || !p || count < 1 || !pbase || !radius || pdim < 1 || maxdim != 2 || !d_pairs || !pair_off ||
+      (p->model & 2))
+    return SSQ_EINVAL;
+  Handle* H = h->h;
+  H->err.clear();
+  if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  if (!H->rc && !(H->rc = ripsco_create())) return SSQ_ENOMEM;
+  return ripsco_reduce(H->rc, &H->eng, count, d_pts, pbase, pdim, radius, *p, d_pairs, pairs_cap, pair_off, 0, stats,
+                       &H->err, nullptr);
 }

@@ -35,4 +35,41 @@ int vr_builder_build_batch(VrBuilder* b, hipStream_t s, int count, const float* 
                            int pdim, const float* radius, int maxdim, const int64_t** d_colptr, const int32_t** d_rows,
                            const int32_t** d_dims, int64_t* n_out, int64_t* nnz_out, int64_t* cell_base);
 
+// What the reduction engine offers a builder that writes a matrix straight into its buffers
+// (Engine::prebuilt_begin / prebuilt_reduce in ssq_engine.hip).
+struct PrebuiltTarget {
+  virtual ~PrebuiltTarget() = default;
+  // cells [0, n) are rows, [0, ncols) also columns (colptr[ncols + 1], dims[ncols]); returns
+  // pool (nnz rows, then arena free entries), leftone[n] (to fill) and negflag[n] (zeroed)
+  virtual int pre_begin(int64_t n, int64_t ncols, int64_t nnz, int64_t arena, const int64_t* colptr,
+                        const int32_t* dims, int32_t maxdim, int32_t** pool, uint32_t** leftone,
+                        uint8_t** negflag) = 0;
+  virtual int pre_reduce(const ssq_params& p, ssq_stats* st, bool* oom) = 0;
+  virtual const int32_t* pre_low() const = 0;
+  virtual const uint32_t* pre_piv() const = 0;
+  virtual hipStream_t pre_stream() const = 0;
+  virtual const std::string& pre_error() const = 0;
+};
+
+// Batched Vietoris-Rips cohomology (dimension <= 2, clouds of at most kRipsMaxPoints points)
+// built straight into the engine's dual layout (ssq_ripsco.hip).
+constexpr int kRipsMaxPoints = 4096;
+struct RipsCo;
+RipsCo* ripsco_create();
+void ripsco_destroy(RipsCo* b);
+// Exact edge and triangle counts of each cloud (device points, host bases and thresholds).
+int ripsco_count(RipsCo* b, hipStream_t s, int count, const float* d_pts, const int64_t* pbase, int pdim,
+                 const float* radius, int64_t* edges, int64_t* triangles);
+// Pairs of every cloud, sorted by birth, in the indices of its own (filtration, dim, lex)
+// ordered complex: d_pairs[2 * pair_off[c] ...] for cloud c (pair_off: count + 1 entries).
+// Clouds are reduced in groups sized to the free device memory (mem_budget 0 = automatic).
+// With d_pairs null the pairs go to a buffer of the builder's, returned in *d_pairs_owned
+// (valid until its next call).
+int ripsco_reduce(RipsCo* b, PrebuiltTarget* eng, int count, const float* d_pts, const int64_t* pbase, int pdim,
+                  const float* radius, const ssq_params& p, int64_t* d_pairs, int64_t pairs_cap, int64_t* pair_off,
+                  int64_t mem_budget, ssq_stats* st, std::string* err, const int64_t** d_pairs_owned);
+// Triangle count of each cloud from the last ripsco_count / ripsco_reduce call.
+const int64_t* ripsco_triangles(const RipsCo* b);
+const int64_t* ripsco_edges(const RipsCo* b);
+
 }  // namespace ssq

@@ -124,6 +124,17 @@ __device__ __forceinline__ int wave_xor_lds(const int32_t* A, int a, const int32
   return r;
 }
 
+// Rips edge length: sqrt of the float64 sum of squared coordinate differences, summed in
+// coordinate order and rounded once to float32 (the host builder's definition, ssq_gen.cpp).
+__device__ __forceinline__ float rips_dist(const float* P, int64_t i, int64_t j, int pdim) {
+  double acc = 0.0;
+  for (int k = 0; k < pdim; k++) {
+    const double d = (double)P[i * pdim + k] - (double)P[j * pdim + k];
+    acc += d * d;
+  }
+  return (float)sqrt(acc);
+}
+
 __device__ __forceinline__ void wave_copy(int32_t* dst, const int32_t* src, int n) {
   for (int t = lane_id(); t < n; t += kWave) dst[t] = src[t];
 }

@@ -30,12 +30,7 @@ namespace {
 constexpr int kVrBlock = 256;
 
 __device__ __forceinline__ float vr_dist(const float* P, int64_t i, int64_t j, int pdim) {
-  double acc = 0.0;
-  for (int k = 0; k < pdim; k++) {
-    const double d = (double)P[i * pdim + k] - (double)P[j * pdim + k];
-    acc += d * d;
-  }
-  return (float)sqrt(acc);
+  return rips_dist(P, i, j, pdim);
 }
 
 __global__ void k_vr_set(int64_t* p, int64_t v) { *p = v; }

@@ -84,6 +84,10 @@ typedef struct ssq_stats {
   double t_add_long_kernel_ms; /* summed duration of the long-column kernel launches     */
   int64_t max_chain;      /* most additions one column made in one round                */
   int64_t max_addition_len; /* largest |a|+|b| of one addition                          */
+  int64_t n_huge_additions; /* additions into a device-memory bit set (columns whose rows
+                               spread past the LDS window)                                */
+  double huge_bytes;      /* algorithmic bytes of those additions                          */
+  double t_add_huge_kernel_ms; /* summed duration of the huge-column kernel launches      */
 } ssq_stats;
 
 /* ---- I/O (main.cu:90-167 processFiles; main.cu:191-229 writeVectorToFile) ---------- */

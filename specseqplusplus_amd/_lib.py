@@ -38,7 +38,9 @@ class ssq_stats(ctypes.Structure):
                 ("n_add_launches", ctypes.c_int64), ("n_tiled_additions", ctypes.c_int64),
                 ("n_dense_additions", ctypes.c_int64), ("dense_bytes", ctypes.c_double),
                 ("t_add_long_kernel_ms", ctypes.c_double),
-                ("max_chain", ctypes.c_int64), ("max_addition_len", ctypes.c_int64)]
+                ("max_chain", ctypes.c_int64), ("max_addition_len", ctypes.c_int64),
+                ("n_huge_additions", ctypes.c_int64), ("huge_bytes", ctypes.c_double),
+                ("t_add_huge_kernel_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <string>
 #include <atomic>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -90,7 +91,9 @@ struct Dev {
   uint8_t* flag;       // scratch per-cell flag (clear / negative)
   uint32_t* piv;       // row -> column
   uint32_t* leftone;   // row -> leftmost column containing it
-  int32_t* work[5];    // W0, W1, A0, A1, long list
+  int32_t* work[6];    // W0, W1, A0, A1, long list, huge list
+  uint32_t* gbits;     // huge-column bit sets: one region of gspan bits per k_add_huge workgroup
+  int32_t gspan;       // rows covered by one region
   uint32_t* ctr;       // [0..1] cW, [2..3] cA, [4] apparent, [5] hd-cleared
   unsigned long long* arena_used;  // ints used past arena_base
   int64_t arena_base, arena_cap;
@@ -110,7 +113,9 @@ constexpr int kA = 2;          // [2..3] add-list counts
 constexpr int kApparent = 4;   // apparent pairs found by the init passes
 constexpr int kHdCleared = 5;  // cells cleared by guided clearing
 constexpr int kLong = 6;       // long-column list count (this round)
+constexpr int kHuge = 7;       // huge-column list count (this round)
 constexpr int kFlags = 8;      // [8..9] input validation flags
+constexpr int kHugeNext = 10;  // next huge-list column to take (this round)
 constexpr int kRowStat = 16;   // [16..23] max low+1 per dim, [24..31] min first row per dim
 constexpr int kStats = 32;     // 8 x u64: additions, bytes, tiled additions, dense additions, dense bytes
 constexpr int kArena = 48;     // u64: arena entries used
@@ -493,7 +498,9 @@ __global__ void k_claim(Dev d, int r) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     d.ctr[cw::kA + nx] = 0;
     d.ctr[cw::kW + nx] = 0;
-    d.ctr[cw::kLong] = 0;  // the long list of the previous round is consumed
+    d.ctr[cw::kLong] = 0;  // the long and huge lists of the previous round are consumed
+    d.ctr[cw::kHuge] = 0;
+    d.ctr[cw::kHugeNext] = 0;
   }
   const uint32_t cnt = d.ctr[cw::kW + in];
   const int32_t* list = d.work[in];
@@ -703,6 +710,8 @@ __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t
 constexpr int kLongBits = 1 << 19;
 constexpr int kLongWords = kLongBits / 32;
 constexpr int kLongThreads = 512;
+constexpr int kHugeGroups = 256;        // k_add_huge workgroups, one device bit set region each
+constexpr int32_t kHugeBits = 1 << 27;  // rows of one region (16 MiB)
 
 __device__ __forceinline__ int32_t block_max_i32(int32_t v, int32_t* scratch) {
   v = wave_max(v);
@@ -754,7 +763,6 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* scratch, 
 
 __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t minrow) {
   __shared__ uint32_t bits[kLongWords];
-  __shared__ int32_t stage[3][kTileBuf];
   __shared__ int32_t red[2 * (kLongThreads / kWave)];
   __shared__ int32_t sh_count, sh_flag;
   __shared__ int64_t sh_off;
@@ -772,6 +780,7 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
     int32_t a_cap = d.cap[c];
     int32_t a_len = d.len[c];
     int32_t l = d.low[c];
+    bool to_huge = false;
     const int32_t lo = l - d.long_window + 1;  // window [lo, l] of the bit set
     bool dense = d.pool[a_off] >= lo;
     bool failed = false;
@@ -936,31 +945,9 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
         a_len = r_len;
         l = r_len ? newl : -1;
       } else {
-        // tiled merge by wave 0 into a fresh slot
-        if (tid == 0) {
-          int64_t o;
-          int32_t cp;
-          sh_flag = store_alloc(d, a_len + b_len, &o, &cp) ? 1 : 0;
-          sh_off = o;
-          sh_cap = cp;
-        }
-        __syncthreads();
-        if (!sh_flag) { failed = true; break; }
-        if (tid < kWave)
-          r_len = wave_tiled_xor(d.pool + a_off, a_len, B, b_len, d.pool + sh_off, stage[0], stage[1], stage[2]);
-        if (tid == 0) sh_count = r_len;
-        __syncthreads();
-        r_len = sh_count;
-        n_tiled += 1;
-        n_add += 1;
-        n_dbytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
-        max_len = max(max_len, (uint32_t)(a_len + b_len));
-        chain += 1;
-        a_off = sh_off;
-        a_cap = sh_cap;
-        a_len = r_len;
-        l = r_len ? d.pool[a_off + r_len - 1] : -1;
-        __syncthreads();
+        // the column's rows leave the window: k_add_huge continues the chase in this round
+        to_huge = true;
+        break;
       }
       if (l < 0 || l < minrow) break;
     }
@@ -1010,7 +997,10 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
         d.len[c] = a_len;
         d.low[c] = a_len ? l : -1;
         d.state[c] = a_len ? ST_PENDING : ST_ZERO;
-        if (a_len && l >= minrow) {
+        if (to_huge) {
+          const uint32_t pos = atomicAdd(&d.ctr[cw::kHuge], 1u);
+          d.work[5][pos] = c;
+        } else if (a_len && l >= minrow) {
           const uint32_t p = d.piv[l];
           if (p == kEmpty || p > (uint32_t)c) {
             const uint32_t pos = atomicAdd(&d.ctr[cw::kW + nx], 1u);
@@ -1026,6 +1016,248 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
     atomicAdd(&d.stats[2], n_tiled);
     atomicAdd(&d.stats[3], n_dense);
     atomicAdd(&d.stats[4], n_dbytes);
+    atomicMax(&d.ctr[cw::kChain], max_chain);
+    atomicMax(&d.ctr[cw::kChain + 1], max_len);
+  }
+}
+
+// Columns whose rows spread past the LDS window (kLongBits rows): one workgroup per column,
+// the running column kept as a bit set in this workgroup's region of device memory (gspan
+// rows below the column's low), toggled by global atomics.  A region is all zero between
+// columns: reading the column back clears the words it read.  A pivot with rows below the
+// region sends the column to sorted form and tiled merges, ping-ponging between the column's
+// own slot and one scratch slot.
+__device__ __forceinline__ uint32_t gload(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // past the L1, after atomics
+}
+
+// The column back from its region in sorted form (words [w_lo, w_hi]), each word read is
+// cleared.  Returns false when the store is exhausted.
+__device__ bool huge_write_back(Dev& d, uint32_t* gb, int32_t lo, int32_t w_lo, int32_t w_hi, int64_t* a_off,
+                                int32_t* a_cap, int32_t* a_len, int32_t* red, int32_t* sh_flag, int64_t* sh_off,
+                                int32_t* sh_cap) {
+  const int tid = threadIdx.x;
+  const int32_t W = w_hi >= w_lo ? w_hi - w_lo + 1 : 0;
+  const int32_t per = (W + kLongThreads - 1) / kLongThreads;
+  const int32_t v0 = w_lo + min(W, tid * per), v1 = w_lo + min(W, tid * per + per);
+  int32_t pc = 0;
+  for (int32_t w = v0; w < v1; w++) pc += __popc(gload(gb + w));
+  int32_t total;
+  const int32_t pos0 = block_excl_scan(pc, red, &total);
+  if (tid == 0) {
+    int64_t o = *a_off;
+    int32_t cp = *a_cap;
+    *sh_flag = 1;
+    if (total > cp && !store_alloc(d, total, &o, &cp)) *sh_flag = 0;
+    *sh_off = o;
+    *sh_cap = cp;
+  }
+  __syncthreads();
+  const bool ok = *sh_flag != 0;
+  int32_t* O = d.pool + *sh_off;
+  int32_t pos = pos0;
+  for (int32_t w = v0; w < v1; w++) {
+    uint32_t x = gload(gb + w);
+    if (!x) continue;
+    gb[w] = 0u;
+    if (ok)
+      while (x) {
+        const int b = __ffs(x) - 1;
+        O[pos++] = lo + (w << 5) + b;
+        x &= x - 1;
+      }
+  }
+  __threadfence();
+  __syncthreads();
+  if (ok) {
+    *a_off = *sh_off;
+    *a_cap = *sh_cap;
+    *a_len = total;
+  }
+  return ok;
+}
+
+__global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t minrow) {
+  __shared__ int32_t stage[3][kTileBuf];
+  __shared__ int32_t red[2 * (kLongThreads / kWave)];
+  __shared__ int32_t sh_count, sh_flag, sh_minw, sh_cap;
+  __shared__ int64_t sh_off;
+  __shared__ uint32_t sh_item;
+  const int nx = (r + 1) & 1;
+  const int tid = threadIdx.x;
+  const uint32_t cnt = d.ctr[cw::kHuge];
+  const int32_t* list = d.work[5];
+  uint32_t* gb = d.gbits + (int64_t)blockIdx.x * (d.gspan >> 5);
+  unsigned long long n_add = 0, n_tiled = 0, n_bytes = 0;
+  uint32_t max_chain = 0, max_len = 0;
+  for (;;) {  // columns taken one at a time: their chases differ in length by orders of magnitude
+    if (tid == 0) sh_item = atomicAdd(&d.ctr[cw::kHugeNext], 1u);
+    __syncthreads();
+    const uint32_t it = sh_item;
+    if (it >= cnt) break;
+    const int32_t c = list[it];
+    int64_t a_off = d.off[c];
+    int32_t a_cap = d.cap[c];
+    int32_t a_len = d.len[c];
+    int32_t l = d.low[c];
+    const int32_t lo = l - d.gspan + 1;  // the region holds rows [lo, l]
+    bool dense = a_len > 0 && d.pool[a_off] >= lo;
+    bool failed = false;
+    uint32_t chain = 0;
+    int64_t t_off = 0;  // scratch slot of the tiled merges (thread 0's copy is used)
+    int32_t t_cap = 0;
+    if (dense) {
+      const int32_t* A = d.pool + a_off;
+      for (int32_t k = tid; k < a_len; k += kLongThreads) {
+        const int32_t x = A[k] - lo;
+        atomicOr(gb + (x >> 5), 1u << (x & 31));
+      }
+      if (tid == 0) {
+        sh_count = a_len;
+        sh_minw = (A[0] - lo) >> 5;
+      }
+      __threadfence();
+      __syncthreads();
+    }
+    for (;;) {
+      l = __builtin_amdgcn_readfirstlane(l);
+      const uint32_t p = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.piv[l]);
+      if (p == kEmpty || p >= (uint32_t)c) break;
+      const PivMeta pm = d.pivmeta[l];
+      const int32_t b_len = __builtin_amdgcn_readfirstlane(pm.len);
+      const int32_t b_first = __builtin_amdgcn_readfirstlane(pm.first);
+      const int64_t b_off = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(pm.off & 0xffffffff)) |
+                            ((int64_t)__builtin_amdgcn_readfirstlane((int)(pm.off >> 32)) << 32);
+      const int32_t* B = d.pool + b_off;
+      if (dense && b_first < lo) {  // leave the region: sorted form, then tiled merges
+        if (!huge_write_back(d, gb, lo, sh_minw, (l - lo) >> 5, &a_off, &a_cap, &a_len, red, &sh_flag, &sh_off,
+                             &sh_cap)) {
+          failed = true;
+          break;
+        }
+        dense = false;
+      }
+      int32_t r_len;
+      if (dense) {
+        int32_t delta = 0;
+        for (int32_t k = tid; k < b_len; k += kLongThreads) {
+          const int32_t x = B[k] - lo;
+          const uint32_t m = 1u << (x & 31);
+          const uint32_t old = atomicXor(gb + (x >> 5), m);
+          delta += (old & m) ? -1 : 1;
+        }
+        for (int k = kWave / 2; k > 0; k >>= 1) delta += __shfl_xor(delta, k, kWave);
+        if (lane_id() == 0) atomicAdd(&sh_count, delta);
+        if (tid == 0) atomicMin(&sh_minw, (b_first - lo) >> 5);
+        __threadfence();
+        __syncthreads();
+        r_len = sh_count;
+        // new low: the highest set bit at or below the old low, wavefront 0 scanning down
+        // 64 words per step
+        if (tid < kWave) {
+          int32_t top = -1;
+          for (int32_t base = r_len > 0 ? (l - lo) >> 5 : -1; base >= 0; base -= kWave) {
+            const int32_t w = base - (int32_t)tid;
+            const uint32_t m = w >= 0 ? gload(gb + w) : 0u;
+            const uint64_t nz = __ballot(m != 0);
+            if (nz) {
+              const int fl = __ffsll((unsigned long long)nz) - 1;
+              const uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, fl);
+              top = lo + ((base - fl) << 5) + (31 - __clz(mm));
+              break;
+            }
+          }
+          if (tid == 0) red[0] = top;
+        }
+        __syncthreads();
+        const int32_t newl = red[0];
+        n_add += 1;
+        n_bytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
+        max_len = max(max_len, (uint32_t)(a_len + b_len));
+        chain += 1;
+        a_len = r_len;
+        l = r_len ? newl : -1;
+        __syncthreads();  // red[0] read by all before the next scan writes it
+      } else {
+        // tiled merge by wave 0 into the other of two slots: the column's own and one scratch
+        // slot, swapped after each addition; the scratch slot is replaced (twice the need)
+        // only when a merge would not fit
+        if (tid == 0) {
+          sh_flag = 1;
+          const int32_t need = a_len + b_len;
+          if (need > t_cap) {
+            int64_t o;
+            int32_t cp;
+            const int32_t want = need <= 0x3fffffff ? 2 * need : need;
+            sh_flag = store_alloc(d, want, &o, &cp) ? 1 : 0;
+            t_off = o;
+            t_cap = cp;
+          }
+          sh_off = t_off;
+          sh_cap = t_cap;
+        }
+        __syncthreads();
+        if (!sh_flag) {
+          failed = true;
+          break;
+        }
+        if (tid < kWave)
+          r_len = wave_tiled_xor(d.pool + a_off, a_len, B, b_len, d.pool + sh_off, stage[0], stage[1], stage[2]);
+        if (tid == 0) {
+          sh_count = r_len;
+          t_off = a_off;  // the column's previous storage becomes the scratch slot
+          t_cap = a_cap;
+        }
+        __syncthreads();
+        r_len = sh_count;
+        n_tiled += 1;
+        n_add += 1;
+        n_bytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
+        max_len = max(max_len, (uint32_t)(a_len + b_len));
+        chain += 1;
+        a_off = sh_off;
+        a_cap = sh_cap;
+        a_len = r_len;
+        l = r_len ? d.pool[a_off + r_len - 1] : -1;
+        __syncthreads();
+      }
+      if (l < 0 || l < minrow) break;
+    }
+    max_chain = max(max_chain, chain);
+    if (dense && !failed && a_len > 0) {
+      if (!huge_write_back(d, gb, lo, sh_minw, (l - lo) >> 5, &a_off, &a_cap, &a_len, red, &sh_flag, &sh_off,
+                           &sh_cap))
+        failed = true;
+    } else if (dense && failed && a_len > 0) {  // keep the region zero for the next column
+      huge_write_back(d, gb, lo, sh_minw, (l - lo) >> 5, &a_off, &a_cap, &a_len, red, &sh_flag, &sh_off, &sh_cap);
+    }
+    if (tid == 0) {
+      if (failed) {
+        d.len[c] = 0;
+        d.low[c] = -1;
+        d.state[c] = ST_ZERO;
+      } else {
+        d.off[c] = a_off;
+        d.cap[c] = a_cap;
+        d.len[c] = a_len;
+        d.low[c] = a_len ? l : -1;
+        d.state[c] = a_len ? ST_PENDING : ST_ZERO;
+        if (a_len && l >= minrow) {
+          const uint32_t p = d.piv[l];
+          if (p == kEmpty || p > (uint32_t)c) {
+            const uint32_t pos = atomicAdd(&d.ctr[cw::kW + nx], 1u);
+            d.work[nx][pos] = c;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && n_add) {
+    atomicAdd(&d.stats[0], n_add);
+    atomicAdd(&d.stats[2], n_tiled);
+    atomicAdd(&d.stats[5], n_add - n_tiled);
+    atomicAdd(&d.stats[6], n_bytes);
     atomicMax(&d.ctr[cw::kChain], max_chain);
     atomicMax(&d.ctr[cw::kChain + 1], max_len);
   }
@@ -1709,6 +1941,8 @@ class Engine : public PrebuiltTarget {
     if (const char* e = std::getenv("SSQ_TEST_SHORT_MAX")) short_max_ = std::max(2, std::min(kTile, std::atoi(e)));
     if (const char* e = std::getenv("SSQ_TEST_WINDOW_BITS"))
       long_window_ = std::max(64, std::min(kLongBits, std::atoi(e)));
+    if (const char* e = std::getenv("SSQ_TEST_GLOBAL_BITS"))  // small regions push columns to tiled merges
+      gspan_ = std::max(64, std::min(kHugeBits, std::atoi(e) & ~31));
     if (timing_)
       for (auto& e : ev_) HIPCHK(hipEventCreate(&e));
     return SSQ_OK;
@@ -1754,6 +1988,7 @@ class Engine : public PrebuiltTarget {
   const int32_t* pre_low() const override { return pre_.low; }
   const uint32_t* pre_piv() const override { return pre_.piv; }
   hipStream_t pre_stream() const override { return stream_; }
+  int pre_device() const override { return device_; }
   const std::string& pre_error() const override { return err_; }
 
  private:
@@ -1781,7 +2016,7 @@ class Engine : public PrebuiltTarget {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
                       &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
                       &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_,
-                      &b_scrc_, &b_kbase_, &b_skey_, &b_stmp_})
+                      &b_scrc_, &b_kbase_, &b_skey_, &b_stmp_, &b_gbits_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
     if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
@@ -1826,6 +2061,23 @@ class Engine : public PrebuiltTarget {
   int init_lists(Dev& d, int32_t maxdim, const uint8_t* hd, bool neg_known, const ssq_params& p, LoopPlan* L,
                  ssq_stats* S);
   int page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPlan& L, ssq_stats* S, bool* oom_out);
+  // The huge-column bit set regions, allocated and zeroed once (k_add_huge leaves them zero).
+  int ensure_gbits(Dev* d) {
+    const size_t words = (size_t)kHugeGroups * (size_t)(gspan_ >> 5);
+    if (!gbits_ready_) {
+      uint32_t* g;
+      int rc = ensure(b_gbits_, words, &g);
+      if (rc) return rc;
+      HIPCHK(hipMemsetAsync(g, 0, words * sizeof(uint32_t), stream_));
+      gbits_ready_ = true;
+    }
+    d->gbits = (uint32_t*)b_gbits_.p;
+    d->gspan = gspan_;
+    return SSQ_OK;
+  }
+  DevBuf b_gbits_;
+  bool gbits_ready_ = false;
+  int32_t gspan_ = kHugeBits;
   Dev pre_{};                // the prebuilt matrix between prebuilt_begin and prebuilt_reduce
   int32_t pre_maxdim_ = 0;
   int build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int64_t* colptr, const int32_t* rows,
@@ -2247,10 +2499,11 @@ int Engine::page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPla
   // page loop, dimensions high to low (specseq.h:598-704)
   const int64_t upend = L.upend;
   int32_t* W0;
-  int rc = ensure(b_work_, (size_t)(5 * std::max<int64_t>(upend, 1) + 64), &W0);
+  int rc = ensure(b_work_, (size_t)(6 * std::max<int64_t>(upend, 1) + 64), &W0);
   if (rc) return rc;
+  if ((rc = ensure_gbits(&d))) return rc;
   const int64_t wcap = std::max<int64_t>(upend, 1);
-  for (int k = 0; k < 5; k++) d.work[k] = W0 + k * wcap;
+  for (int k = 0; k < 6; k++) d.work[k] = W0 + k * wcap;
   const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * add_waves_per_cu_);
   const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
   const int rounds_per_sync = rounds_per_sync_;
@@ -2283,20 +2536,24 @@ int Engine::page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPla
             if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 1], stream_));
             hipLaunchKernelGGL(k_add_long, dim3(long_grid), dim3(kLongThreads), 0, stream_, d, r, fl);
             if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 2], stream_));
+            hipLaunchKernelGGL(k_add_huge, dim3(kHugeGroups), dim3(kLongThreads), 0, stream_, d, r, fl);
+            if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 3], stream_));
           }
           HIPCHK(hipGetLastError());
           if ((rc = sync_read(d.ctr, cw::kWords))) return rc;
           S->n_add_launches += rounds_per_sync;
           if (timing_)
             for (int t = 0; t < rounds_per_sync; t++) {
-              float ms = 0.f, ms2 = 0.f;
+              float ms = 0.f, ms2 = 0.f, ms3 = 0.f;
               HIPCHK(hipEventElapsedTime(&ms, ev_[4 * t], ev_[4 * t + 1]));
               HIPCHK(hipEventElapsedTime(&ms2, ev_[4 * t + 1], ev_[4 * t + 2]));
+              HIPCHK(hipEventElapsedTime(&ms3, ev_[4 * t + 2], ev_[4 * t + 3]));
               S->t_add_kernel_ms += ms;
               S->t_add_long_kernel_ms += ms2;
+              S->t_add_huge_kernel_ms += ms3;
               if (trace_)
-                std::fprintf(stderr, "[ssq] dim %d page %lld round %d: k_add %.3f ms, k_add_long %.3f ms\n", dim,
-                             (long long)S->n_pages, r - rounds_per_sync + t, ms, ms2);
+                std::fprintf(stderr, "[ssq] dim %d page %lld round %d: k_add %.3f ms, k_add_long %.3f ms, k_add_huge %.3f ms\n",
+                             dim, (long long)S->n_pages, r - rounds_per_sync + t, ms, ms2, ms3);
             }
           if (trace_)
             std::fprintf(stderr,
@@ -2514,6 +2771,8 @@ void Engine::fill_stats(ssq_stats* S, int64_t nnz, int64_t arena_cap) {
   S->n_tiled_additions = (int64_t)s64[2];
   S->n_dense_additions = (int64_t)s64[3];
   S->dense_bytes = (double)s64[4];
+  S->n_huge_additions = (int64_t)s64[5];
+  S->huge_bytes = (double)s64[6];
   S->max_chain = host_ctr_[cw::kChain];
   S->max_addition_len = host_ctr_[cw::kChain + 1];
 }
@@ -2606,11 +2865,15 @@ struct Handle {
   PinBuf pin_in, pin_out;
   VrBuilder* vr = nullptr;
   RipsCo* rc = nullptr;
+  // further engines and builders of the batched Rips path: its groups run on several streams
+  std::vector<std::unique_ptr<Engine>> xeng;
+  std::vector<RipsCo*> xrc;
   DevBuf pts;
   explicit Handle(int dev) : eng(dev) {}
   ~Handle() {
     if (vr) vr_builder_destroy(vr);
     if (rc) ripsco_destroy(rc);
+    for (RipsCo* b : xrc) ripsco_destroy(b);
     if (pts.p) (void)hipFree(pts.p);
     for (DevBuf* b : {&in_colptr, &in_rows, &in_dims, &out_pairs, &out_ess})
       if (b->p) (void)hipFree(b->p);
@@ -2944,6 +3207,44 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
   return SSQ_OK;
 }
 
+// Streams of the batched Rips path: SSQ_RIPS_STREAMS (1-4), default 3.  HIP places a stream on
+// one of GPU_MAX_HW_QUEUES hardware queues (4 by default), so at most 4.
+static int rips_streams() {
+  int n = 3;
+  if (const char* e = std::getenv("SSQ_RIPS_STREAMS")) n = std::atoi(e);
+  return std::max(1, std::min(4, n));
+}
+
+static int rips_reduce(Handle* H, int count, const float* d_pts, const int64_t* pbase, int pdim, const float* radius,
+                       const ssq_params& p, int64_t* d_pairs, int64_t pairs_cap, int64_t* pair_off, ssq_stats* stats,
+                       const int64_t** d_pairs_owned) {
+  if (!H->rc && !(H->rc = ripsco_create())) return SSQ_ENOMEM;
+  const int ns = rips_streams();
+  while ((int)H->xeng.size() < ns - 1) {
+    std::unique_ptr<Engine> e(new (std::nothrow) Engine(H->eng.device()));
+    RipsCo* b = ripsco_create();
+    if (!e || !b) {
+      if (b) ripsco_destroy(b);
+      return SSQ_ENOMEM;
+    }
+    if (int rc = e->init()) {
+      ripsco_destroy(b);
+      H->err = e->error();
+      return rc;
+    }
+    H->xeng.push_back(std::move(e));
+    H->xrc.push_back(b);
+  }
+  std::vector<RipsCo*> bs{H->rc};
+  std::vector<PrebuiltTarget*> es{&H->eng};
+  for (int k = 0; k < ns - 1; k++) {
+    bs.push_back(H->xrc[(size_t)k]);
+    es.push_back(H->xeng[(size_t)k].get());
+  }
+  return ripsco_reduce(bs.data(), es.data(), ns, count, d_pts, pbase, pdim, radius, p, d_pairs, pairs_cap, pair_off, 0,
+                       stats, &H->err, d_pairs_owned);
+}
+
 // Point clouds through the batched Rips coboundary builder: the points go to the device,
 // the pairs come back per cloud, and the essential cells are the complement of the pairs in
 // each cloud's n + edges + triangles cells.
@@ -2966,11 +3267,9 @@ static int points_cohomology(Handle* H, int count, const float* const* pts, cons
     H->err = "host to device copy failed";
     return SSQ_EDEVICE;
   }
-  if (!H->rc && !(H->rc = ripsco_create())) return SSQ_ENOMEM;
   std::vector<int64_t> off((size_t)count + 1, 0);
   const int64_t* dp = nullptr;
-  rc = ripsco_reduce(H->rc, &H->eng, count, (const float*)H->pts.p, pbase.data(), pdim, r, p, nullptr, 0, off.data(),
-                     0, stats, &H->err, &dp);
+  rc = rips_reduce(H, count, (const float*)H->pts.p, pbase.data(), pdim, r, p, nullptr, 0, off.data(), stats, &dp);
   if (rc) return rc;
   const int64_t np = off[count];
   if ((rc = pin_grow(H, H->pin_out, sizeof(int64_t) * std::max<int64_t>(2 * np, 1)))) return rc;
@@ -3097,7 +3396,5 @@ extern "C" int ssq_batch_reduce_points_device(ssq_handle* h, int count, const fl
   Handle* H = h->h;
   H->err.clear();
   if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
-  if (!H->rc && !(H->rc = ripsco_create())) return SSQ_ENOMEM;
-  return ripsco_reduce(H->rc, &H->eng, count, d_pts, pbase, pdim, radius, *p, d_pairs, pairs_cap, pair_off, 0, stats,
-                       &H->err, nullptr);
+  return rips_reduce(H, count, d_pts, pbase, pdim, radius, *p, d_pairs, pairs_cap, pair_off, stats, nullptr);
 }

@@ -48,6 +48,7 @@ struct PrebuiltTarget {
   virtual const int32_t* pre_low() const = 0;
   virtual const uint32_t* pre_piv() const = 0;
   virtual hipStream_t pre_stream() const = 0;
+  virtual int pre_device() const = 0;
   virtual const std::string& pre_error() const = 0;
 };
 
@@ -62,12 +63,16 @@ int ripsco_count(RipsCo* b, hipStream_t s, int count, const float* d_pts, const 
                  const float* radius, int64_t* edges, int64_t* triangles);
 // Pairs of every cloud, sorted by birth, in the indices of its own (filtration, dim, lex)
 // ordered complex: d_pairs[2 * pair_off[c] ...] for cloud c (pair_off: count + 1 entries).
-// Clouds are reduced in groups sized to the free device memory (mem_budget 0 = automatic).
+// Clouds are reduced in groups sized to the free device memory (mem_budget 0 = automatic,
+// shared by the streams).
 // With d_pairs null the pairs go to a buffer of the builder's, returned in *d_pairs_owned
 // (valid until its next call).
-int ripsco_reduce(RipsCo* b, PrebuiltTarget* eng, int count, const float* d_pts, const int64_t* pbase, int pdim,
-                  const float* radius, const ssq_params& p, int64_t* d_pairs, int64_t pairs_cap, int64_t* pair_off,
-                  int64_t mem_budget, ssq_stats* st, std::string* err, const int64_t** d_pairs_owned);
+// Groups run on nstreams engines side by side (builder b[k] with engine eng[k]; b[0] keeps the
+// sizing pass and the output).
+int ripsco_reduce(RipsCo* const* b, PrebuiltTarget* const* eng, int nstreams, int count, const float* d_pts,
+                  const int64_t* pbase, int pdim, const float* radius, const ssq_params& p, int64_t* d_pairs,
+                  int64_t pairs_cap, int64_t* pair_off, int64_t mem_budget, ssq_stats* st, std::string* err,
+                  const int64_t** d_pairs_owned);
 // Triangle count of each cloud from the last ripsco_count / ripsco_reduce call.
 const int64_t* ripsco_triangles(const RipsCo* b);
 const int64_t* ripsco_edges(const RipsCo* b);

@@ -39,10 +39,13 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ssq_internal.h"
@@ -444,8 +447,7 @@ __global__ void __launch_bounds__(kBlock) k_rc_side(Grp q, const int64_t* aoff, 
                                                     const uint16_t* inc_n, const uint64_t* B, const uint16_t* Bp,
                                                     const int64_t* TB, const int32_t* rlo, const int32_t* rhi,
                                                     const int32_t* remap, const int64_t* colptr, const int64_t* cnt,
-                                                    const int32_t* cof, int32_t* stg, int32_t* na, int32_t* nb,
-                                                    uint8_t* needsort) {
+                                                    const int32_t* cof, int32_t* stg, int32_t* na, int32_t* nb) {
   __shared__ uint64_t sw[kRipsMaxPoints];
   __shared__ int32_t stb[kRipsMaxPoints];  // cloud-local TB[m], sign bit: m's run has several edges
   __shared__ uint16_t sp[kRipsMaxPoints];
@@ -482,7 +484,6 @@ __global__ void __launch_bounds__(kBlock) k_rc_side(Grp q, const int64_t* aoff, 
         const bool lower = vl < z;
         const uint64_t below = (1ull << zz) - 1;
         int64_t written = 0;
-        bool tieany = false;
         for (int i0 = iz + 1; i0 < deg; i0 += kWave) {
           const int i = i0 + lane;
           bool hit = false;
@@ -493,10 +494,7 @@ __global__ void __launch_bounds__(kBlock) k_rc_side(Grp q, const int64_t* aoff, 
               hit = true;
               const uint32_t s = (uint32_t)stb[i];
               rho = (int32_t)(s & 0x7fffffffu) + sp[i] + __popcll(x & below);
-              if (s & 0x80000000u) {
-                rho = remap[tbc + rho];
-                tieany = true;
-              }
+              if (s & 0x80000000u) rho = remap[tbc + rho];
             }
           }
           const uint64_t m = __ballot(hit);
@@ -506,11 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_rc_side(Grp q, const int64_t* aoff, 
           }
           written += __popcll(m);
         }
-        const uint64_t anytie = __ballot(tieany);
-        if (lane == 0) {
-          (lower ? na : nb)[pe] = (int32_t)written;
-          if (anytie) needsort[pe] = 1;
-        }
+        if (lane == 0) (lower ? na : nb)[pe] = (int32_t)written;
       }
     }
     __syncthreads();
@@ -518,14 +512,56 @@ __global__ void __launch_bounds__(kBlock) k_rc_side(Grp q, const int64_t* aoff, 
 }
 
 // ---- merge: the coboundary column in increasing dual id (decreasing rank) ------------------
-// One wavefront per edge column: foreign entries (lower-endpoint list ascending, higher-end
-// list stored descending) placed by rank, then the own triangles; each own triangle's leftmost
-// column is this one.  Columns touched by a run of equal-length edges are left to k_rc_msort.
+// One wavefront per edge column, three sorted lists: the lower endpoint's (ascending), the
+// higher endpoint's (stored descending) and the own triangles (ascending: their natural index,
+// through the run's lexicographic remap when the edge's length is shared).  Within a run of
+// equal-length edges the lists stay sorted (the triangles of a fixed vertex pair are
+// lexicographically ordered by the third vertex, and so are a vertex's edges within a run);
+// only the own list interleaves with the foreign entries of its run.  Each entry is placed by
+// counting the larger entries of the other two lists (binary searches).  Each own triangle's
+// leftmost column is this one.
+__device__ __forceinline__ int32_t own_rank(const int32_t* remap, bool tie, int64_t base, int64_t own0, int i) {
+  return tie ? remap[base + own0 + i] : (int32_t)(own0 + i);
+}
+// number of entries > x in an ascending list / a descending list / the own list
+__device__ __forceinline__ int count_gt_asc(const int32_t* v, int n, int32_t x) {
+  int lo = 0, hi = n;  // first index with v > x
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (v[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return n - lo;
+}
+__device__ __forceinline__ int count_gt_desc(const int32_t* v, int n, int32_t x) {
+  int lo = 0, hi = n;  // first index with v < x (v > x before it)
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (v[mid] > x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int count_gt_own(const int32_t* remap, bool tie, int64_t base, int64_t own0, int n,
+                                            int32_t x) {
+  if (!tie) {  // own0 .. own0 + n - 1
+    const int64_t c = own0 + n - 1 - x;
+    return (int)(c < 0 ? 0 : (c > n ? n : c));
+  }
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (remap[base + own0 + mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return n - lo;
+}
+
 __global__ void __launch_bounds__(kBlock) k_rc_merge(Grp q, const int32_t* ecl, const int32_t* rlo, const int32_t* rhi,
                                                      const int64_t* TB, const int32_t* remap, const int64_t* colptr,
                                                      const int64_t* cnt, const int32_t* cof, const int32_t* na,
-                                                     const int32_t* nb, const uint8_t* needsort, const int32_t* stg,
-                                                     int32_t* pool, uint32_t* leftone, uint32_t* err) {
+                                                     const int32_t* nb, const int32_t* stg, int32_t* pool,
+                                                     uint32_t* leftone, uint32_t* err) {
   const int lane = lane_id(), wv = threadIdx.x / kWave;
   const int64_t stg0 = 2 * q.NE;
   for (int64_t p = (int64_t)blockIdx.x * kWaves + wv; p < q.NE; p += (int64_t)gridDim.x * kWaves) {
@@ -539,86 +575,44 @@ __global__ void __launch_bounds__(kBlock) k_rc_merge(Grp q, const int32_t* ecl, 
       if (lane == 0) atomicOr(err, 1u);
       continue;
     }
-    if (needsort[p] || (rhi[p] - rlo[p] > 1 && own > 0)) continue;  // k_rc_msort's
+    const bool tie = own > 0 && rhi[p] - rlo[p] > 1;
     const int32_t* Aa = stg + (c0 - stg0);  // ascending, an entries
     const int32_t* Bd = Aa + (F - bn);      // descending, bn entries
     int32_t* out = pool + c0;
     const int64_t own0 = TB[p] - tbc;
     for (int i = lane; i < an; i += kWave) {
       const int32_t x = Aa[i];
-      int lo = 0, hi = bn;  // first j with Bd[j] < x
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (Bd[mid] > x) lo = mid + 1;
-        else hi = mid;
-      }
-      out[(an - 1 - i) + lo] = (int32_t)tdid(q, g, x);
+      const int pos = (an - 1 - i) + count_gt_desc(Bd, bn, x) + count_gt_own(remap, tie, tbc, own0, own, x);
+      out[pos] = (int32_t)tdid(q, g, x);
     }
     for (int j = lane; j < bn; j += kWave) {
       const int32_t y = Bd[j];
-      int lo = 0, hi = an;  // first i with Aa[i] > y
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (Aa[mid] < y) lo = mid + 1;
-        else hi = mid;
-      }
-      out[j + (an - lo)] = (int32_t)tdid(q, g, y);
+      const int pos = j + count_gt_asc(Aa, an, y) + count_gt_own(remap, tie, tbc, own0, own, y);
+      out[pos] = (int32_t)tdid(q, g, y);
     }
     for (int i = lane; i < own; i += kWave) {
-      const int64_t d = tdid(q, g, own0 + i);
-      out[F + (own - 1 - i)] = (int32_t)d;
+      const int32_t z = own_rank(remap, tie, tbc, own0, i);
+      const int pos = (own - 1 - i) + count_gt_asc(Aa, an, z) + count_gt_desc(Bd, bn, z);
+      const int64_t d = tdid(q, g, z);
+      out[pos] = (int32_t)d;
       leftone[d] = (uint32_t)col;
     }
   }
 }
 
-// Columns with entries from a run of equal-length edges: all ranks gathered in LDS (own ones
-// through the run's lexicographic remap) and sorted descending.  One wavefront per column.
-__global__ void __launch_bounds__(kBlock) k_rc_msort(Grp q, const int32_t* ecl, const int32_t* rlo, const int32_t* rhi,
-                                                     const int64_t* TB, const int32_t* remap, const int64_t* colptr,
-                                                     const int64_t* cnt, const int32_t* cof, const uint8_t* needsort,
-                                                     const int32_t* stg, int32_t* pool, uint32_t* leftone) {
-  __shared__ int32_t sbuf[kWaves][kRipsMaxPoints];
+// Every column's rows strictly increasing and below n (the engine indexes its tables with
+// them): a check of the built matrix before the reduction reads it.  One wavefront per column.
+__global__ void __launch_bounds__(kBlock) k_rc_check(int64_t nc, int64_t n, const int64_t* colptr, const int32_t* pool,
+                                                     uint32_t* err) {
   const int lane = lane_id(), wv = threadIdx.x / kWave;
-  const int64_t stg0 = 2 * q.NE;
-  for (int64_t p = (int64_t)blockIdx.x * kWaves + wv; p < q.NE; p += (int64_t)gridDim.x * kWaves) {
-    const bool tie = rhi[p] - rlo[p] > 1 && cnt[p] > 0;
-    if (!tie && !needsort[p]) continue;
-    const int g = ecl[p];
-    const int64_t e0 = q.eb[g], k = p - e0, tbc = q.tb[g];
-    const int64_t col = edid(q, g, k);
-    const int64_t c0 = colptr[col];
-    const int32_t len = cof[p], own = (int32_t)cnt[p], F = len - own;
-    const int32_t* Aa = stg + (c0 - stg0);
-    int32_t* out = pool + c0;
-    const int64_t own0 = TB[p] - tbc;
-    int32_t* sb = sbuf[wv];
-    for (int i = lane; i < F; i += kWave) sb[i] = Aa[i];
-    for (int i = lane; i < own; i += kWave) {
-      const int32_t r = tie ? remap[tbc + own0 + i] : (int32_t)(own0 + i);
-      sb[F + i] = r;
-      leftone[tdid(q, g, r)] = (uint32_t)col;
+  for (int64_t c = (int64_t)blockIdx.x * kWaves + wv; c < nc; c += (int64_t)gridDim.x * kWaves) {
+    const int64_t o = colptr[c], len = colptr[c + 1] - o;
+    bool bad = false;
+    for (int64_t i = lane; i < len; i += kWave) {
+      const int32_t x = pool[o + i];
+      bad |= x < 0 || x >= n || (i > 0 && pool[o + i - 1] >= x);
     }
-    int P2 = 1;
-    while (P2 < len) P2 <<= 1;
-    for (int i = len + lane; i < P2; i += kWave) sb[i] = -1;
-    wave_sync();
-    for (int kk = 2; kk <= P2; kk <<= 1)  // bitonic, descending
-      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        for (int i = lane; i < P2; i += kWave) {
-          const int ixj = i ^ jj;
-          if (ixj > i) {
-            const int32_t x = sb[i], y = sb[ixj];
-            if ((x < y) == ((i & kk) == 0)) {
-              sb[i] = y;
-              sb[ixj] = x;
-            }
-          }
-        }
-        wave_sync();
-      }
-    for (int i = lane; i < len; i += kWave) out[i] = (int32_t)tdid(q, g, sb[i]);
-    wave_sync();
+    if (__ballot(bad) && lane == 0) atomicOr(err, 2u);
   }
 }
 
@@ -650,16 +644,6 @@ __global__ void k_rc_paircount(Grp q, const int64_t* pos, int64_t* hv, int64_t* 
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < q.G; g += gridDim.x * blockDim.x) {
     hv[g] = pos[q.pb[g + 1]] - pos[q.pb[g]];
     he[g] = pos[q.NV + q.eb[g + 1]] - pos[q.NV + q.eb[g]];
-  }
-}
-__global__ void k_rc_pairbase(int G, const int64_t* hv, const int64_t* he, int64_t* ob) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    int64_t s = 0;
-    for (int g = 0; g < G; g++) {
-      ob[g] = s;
-      s += hv[g] + he[g];
-    }
-    ob[G] = s;
   }
 }
 // primal cell of ranked edge p: n + rank + triangles of smaller filtration
@@ -701,6 +685,36 @@ __global__ void k_rc_pairemit(Grp q, const int32_t* ecl, const int32_t* low, con
   }
 }
 
+// A group's inputs gathered from the sizing pass's arrays (clouds in any order): points,
+// degrees and adjacency bitsets of group cloud g come from its vertex base srcv[g] and bitset
+// base srca[g].
+__global__ void k_rc_gather(Grp q, int pdim, const int64_t* srcv, const int64_t* srca, const float* Pall,
+                            const uint64_t* Aall, const int32_t* degall, const int32_t* upall, float* P, uint64_t* A,
+                            int32_t* deg, int32_t* up) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < q.NV; v += (int64_t)gridDim.x * blockDim.x) {
+    const int g = cloud_of(q.pb, q.G, v);
+    const int64_t vl = v - q.pb[g], sv = srcv[g] + vl;
+    const int W = words_of(q.pb[g + 1] - q.pb[g]);
+    for (int k = 0; k < pdim; k++) P[v * pdim + k] = Pall[sv * pdim + k];
+    deg[v] = degall[sv];
+    up[v] = upall[sv];
+    const uint64_t* a = Aall + srca[g] + vl * W;
+    uint64_t* o = A + q.ab[g] + vl * W;
+    for (int w = 0; w < W; w++) o[w] = a[w];
+  }
+}
+
+// Pairs from per-cloud slots to their packed place: cloud c's cnt[c] pairs move from slot[c]
+// to off[c].  One workgroup per cloud.
+__global__ void k_rc_pack(int C, const int64_t* slot, const int64_t* off, const int64_t* stage, int64_t* out) {
+  for (int c = blockIdx.x; c < C; c += gridDim.x) {
+    const int64_t n = 2 * (off[c + 1] - off[c]);
+    const int64_t* src = stage + 2 * slot[c];
+    int64_t* dst = out + 2 * off[c];
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  }
+}
+
 __global__ void k_rc_set64(int64_t* p, int64_t v) { *p = v; }
 
 // out[n] = out[n - 1] + in[n - 1]: the total after an exclusive scan
@@ -733,9 +747,11 @@ struct RipsCo {
   Buf pb, rad, ab, A, updeg, deg, tri3, temp;
   Buf gtab, eoff, erow, enbr, keys, keys2, vals, vals2, suv, ecl, R, rstart, runincl, runpos, rlo, rhi, lens,
       colptr, aoff, cpoff, cpw, inc_r, inc_n, epos, CP, B, Bp, cnt, cof, TB, tcnt, toff, tiestart, tidx, run_toff,
-      run_rank0, remap, tkeys, tkeys2, tvals, tvals2, na, nb, needsort, dims, flag, pos, hv, he, ob, err, pairs;
+      run_rank0, remap, tkeys, tkeys2, tvals, tvals2, na, nb, dims, flag, pos, hv, he, err, pairs, gpts, gA, gdeg,
+      gup, stage, slots;
   int64_t* host = nullptr;  // pinned scratch for read-backs
   int64_t budget = 0;       // device bytes one group may use
+  bool trace = std::getenv("SSQ_TRACE_GROUPS") != nullptr;  // per-group sizes and times on stderr
   std::vector<int64_t> h_pb, h_ab;
   std::vector<float> h_rad;
   std::vector<int64_t> h_edges, h_tris;
@@ -743,8 +759,8 @@ struct RipsCo {
     for (Buf* b : {&pb, &rad, &ab, &A, &updeg, &deg, &tri3, &temp, &gtab, &eoff, &erow, &enbr, &keys, &keys2, &vals,
                    &vals2, &suv, &ecl, &R, &rstart, &runincl, &runpos, &rlo, &rhi, &lens, &colptr, &aoff, &cpoff, &cpw,
                    &inc_r, &inc_n, &epos, &CP, &B, &Bp, &cnt, &cof, &TB, &tcnt, &toff, &tiestart, &tidx, &run_toff,
-                   &run_rank0, &remap, &tkeys, &tkeys2, &tvals, &tvals2, &na, &nb, &needsort, &dims, &flag, &pos, &hv,
-                   &he, &ob, &err, &pairs})
+                   &run_rank0, &remap, &tkeys, &tkeys2, &tvals, &tvals2, &na, &nb, &dims, &flag, &pos, &hv,
+                   &he, &err, &pairs, &gpts, &gA, &gdeg, &gup, &stage, &slots})
       if (b->p) (void)hipFree(b->p);
     if (host) (void)hipHostFree(host);
   }
@@ -896,6 +912,8 @@ int ripsco_count(RipsCo* B, hipStream_t s, int count, const float* d_pts, const 
 // Cells of one group: a bound well inside the engine's int32 cell ids, and small enough that
 // the per-row tables of two consecutive groups' sizes fit beside each other as buffers grow.
 constexpr int64_t kGroupCells = (int64_t)1 << 30;
+// device bytes each engine keeps for its huge-column bit sets (k_add_huge regions)
+constexpr int64_t kHugeRegionBytes = (int64_t)256 << 24;
 
 // Device bytes of one group, by the buffers below (the engine's included), with the slack of
 // their geometric growth (1.5x for the engine's buffers, 1.25x for the builder's).
@@ -912,38 +930,43 @@ static double group_bytes(int64_t nv, int64_t ne, int64_t nt, int64_t n2, int64_
   return 1.5 * engine + 1.25 * builder + 64.0 * 1048576.0;
 }
 
-static int rc_group(RipsCo* B, PrebuiltTarget* eng, hipStream_t s, int c0, int c1, const float* d_pts, int pdim,
-                    const ssq_params& prm, int64_t* d_out, int64_t out_base, int64_t* pair_off, int64_t* arena_hint,
-                    ssq_stats* acc, std::string* err) {
-  const int G = c1 - c0;
+// One group: clouds[0 .. G) (indices into the sizing pass S), built and reduced on stream s;
+// each cloud's pairs go to stage at its slot, their count to counts[cloud].
+static int rc_group(RipsCo* B, const RipsCo* S, PrebuiltTarget* eng, hipStream_t s, const int* clouds, int G,
+                    const float* d_pts, int pdim, const ssq_params& prm, int64_t* stage, const int64_t* slot,
+                    int64_t* counts, int64_t* arena_hint, ssq_stats* acc, std::string* err) {
+  auto tb0 = std::chrono::steady_clock::now();
   std::vector<int64_t> pb(G + 1), eb(G + 1), rb(G + 1), bb(G + 1), ab(G + 1), tb(G + 1);
-  for (int g = 0; g <= G; g++) {
-    pb[g] = B->h_pb[c0 + g] - B->h_pb[c0];
-    ab[g] = B->h_ab[c0 + g] - B->h_ab[c0];
-  }
-  eb[0] = rb[0] = bb[0] = tb[0] = 0;
+  std::vector<int64_t> srcv(G), srca(G), gslot(G);
+  pb[0] = eb[0] = rb[0] = bb[0] = ab[0] = tb[0] = 0;
   for (int g = 0; g < G; g++) {
-    const int64_t n = pb[g + 1] - pb[g], W = (n + 63) / 64, E = B->h_edges[(size_t)(c0 + g)];
+    const int c = clouds[g];
+    const int64_t n = S->h_pb[c + 1] - S->h_pb[c], W = (n + 63) / 64, E = S->h_edges[(size_t)c];
+    pb[g + 1] = pb[g] + n;
+    ab[g + 1] = ab[g] + n * W;
     eb[g + 1] = eb[g] + E;
     rb[g + 1] = rb[g] + n * n;
     bb[g + 1] = bb[g] + E * W;
-    tb[g + 1] = tb[g] + B->h_tris[(size_t)(c0 + g)];
+    tb[g + 1] = tb[g] + S->h_tris[(size_t)c];
+    srcv[g] = S->h_pb[c];
+    srca[g] = S->h_ab[c];
+    gslot[g] = slot[c];
   }
   const int64_t NV = pb[G], NE = eb[G], NT = tb[G], NC = NV + NE, N = NC + NT;
   const int64_t NNZ = 2 * NE + 3 * NT;
   if (NV == 0) {  // empty clouds only
-    for (int g = 0; g <= G; g++) pair_off[c0 + g] = out_base;
+    for (int g = 0; g < G; g++) counts[clouds[g]] = 0;
     return SSQ_OK;
   }
   if (N >= 0x7fffffffLL) {
     if (err) *err = "group exceeds 2^31 cells";
     return SSQ_EINVAL;
   }
-  // group tables on the device: pb | eb | rb | bb | ab | tb
+  // group tables on the device: pb | eb | rb | bb | ab | tb | srcv | srca | slot
   int64_t* gt;
-  RCGROW(gtab, int64_t, 6 * (G + 1), gt);
+  RCGROW(gtab, int64_t, 9 * (G + 1), gt);
   {
-    std::vector<int64_t> h(6 * (size_t)(G + 1));
+    std::vector<int64_t> h(9 * (size_t)(G + 1), 0);
     for (int g = 0; g <= G; g++) {
       h[g] = pb[g];
       h[(G + 1) + g] = eb[g];
@@ -951,6 +974,11 @@ static int rc_group(RipsCo* B, PrebuiltTarget* eng, hipStream_t s, int c0, int c
       h[3 * (G + 1) + g] = bb[g];
       h[4 * (G + 1) + g] = ab[g];
       h[5 * (G + 1) + g] = tb[g];
+      if (g < G) {
+        h[6 * (G + 1) + g] = srcv[g];
+        h[7 * (G + 1) + g] = srca[g];
+        h[8 * (G + 1) + g] = gslot[g];
+      }
     }
     RCCHK(hipMemcpyAsync(gt, h.data(), sizeof(int64_t) * h.size(), hipMemcpyHostToDevice, s));
   }
@@ -965,10 +993,18 @@ static int rc_group(RipsCo* B, PrebuiltTarget* eng, hipStream_t s, int c0, int c
   q.bb = gt + 3 * (G + 1);
   q.ab = gt + 4 * (G + 1);
   q.tb = gt + 5 * (G + 1);
-  const float* P = d_pts + B->h_pb[c0] * pdim;
-  const uint64_t* A = (const uint64_t*)B->A.p + B->h_ab[c0];
-  const int32_t* updeg = (const int32_t*)B->updeg.p + B->h_pb[c0];
-  const int32_t* deg = (const int32_t*)B->deg.p + B->h_pb[c0];
+  const int64_t* d_srcv = gt + 6 * (G + 1);
+  const int64_t* d_srca = gt + 7 * (G + 1);
+  const int64_t* d_slot = gt + 8 * (G + 1);
+  float* P;
+  uint64_t* A;
+  int32_t *deg, *updeg;
+  RCGROW(gpts, float, NV * pdim, P);
+  RCGROW(gA, uint64_t, ab[G], A);
+  RCGROW(gdeg, int32_t, NV, deg);
+  RCGROW(gup, int32_t, NV, updeg);
+  hipLaunchKernelGGL(k_rc_gather, dim3(grid_of(NV)), dim3(kBlock), 0, s, q, pdim, d_srcv, d_srca, d_pts,
+                     (const uint64_t*)S->A.p, (const int32_t*)S->deg.p, (const int32_t*)S->updeg.p, P, A, deg, updeg);
   const int64_t MAXE = std::max<int64_t>(NE, 1);
 
   // ---- edges: lexicographic fill, then one stable radix sort by (cloud, length) ----
@@ -978,7 +1014,6 @@ static int rc_group(RipsCo* B, PrebuiltTarget* eng, hipStream_t s, int c0, int c
   uint64_t *keys, *keys2, *CP, *Bq;
   uint32_t* suv;
   uint16_t *inc_n, *Bp;
-  uint8_t* needsort;
   RCGROW(eoff, int64_t, NV + 1, eoff);
   RCGROW(erow, int32_t, MAXE, erow);
   RCGROW(enbr, int32_t, MAXE, enbr);
@@ -1134,37 +1169,35 @@ static int rc_group(RipsCo* B, PrebuiltTarget* eng, hipStream_t s, int c0, int c
   // ---- coboundary columns ----
   RCGROW(na, int32_t, MAXE, na);
   RCGROW(nb, int32_t, MAXE, nb);
-  RCGROW(needsort, uint8_t, MAXE, needsort);
   uint32_t* derr;
   RCGROW(err, uint32_t, 4, derr);
   RCCHK(hipMemsetAsync(na, 0, sizeof(int32_t) * MAXE, s));
   RCCHK(hipMemsetAsync(nb, 0, sizeof(int32_t) * MAXE, s));
-  RCCHK(hipMemsetAsync(needsort, 0, MAXE, s));
   RCCHK(hipMemsetAsync(derr, 0, sizeof(uint32_t), s));
   if (NE > 0) {
     hipLaunchKernelGGL(k_rc_side, dim3((unsigned)std::min<int64_t>(NV, 65536)), dim3(kBlock), 0, s, q,
                        (const int64_t*)aoff, (const int32_t*)inc_r, (const uint16_t*)inc_n, (const uint64_t*)Bq,
                        (const uint16_t*)Bp, (const int64_t*)TB, (const int32_t*)rlo, (const int32_t*)rhi,
                        (const int32_t*)remap, (const int64_t*)colptr, (const int64_t*)cnt, (const int32_t*)cof, stg,
-                       na, nb, needsort);
+                       na, nb);
     hipLaunchKernelGGL(k_rc_merge, dim3(grid_of(NE, kWaves)), dim3(kBlock), 0, s, q, (const int32_t*)ecl,
                        (const int32_t*)rlo, (const int32_t*)rhi, (const int64_t*)TB, (const int32_t*)remap,
                        (const int64_t*)colptr, (const int64_t*)cnt, (const int32_t*)cof, (const int32_t*)na,
-                       (const int32_t*)nb, (const uint8_t*)needsort, (const int32_t*)stg, pool, leftone, derr);
-    if (ntie > 0)
-      hipLaunchKernelGGL(k_rc_msort, dim3(grid_of(NE, kWaves)), dim3(kBlock), 0, s, q, (const int32_t*)ecl,
-                         (const int32_t*)rlo, (const int32_t*)rhi, (const int64_t*)TB, (const int32_t*)remap,
-                         (const int64_t*)colptr, (const int64_t*)cnt, (const int32_t*)cof, (const uint8_t*)needsort,
-                         (const int32_t*)stg, pool, leftone);
+                       (const int32_t*)nb, (const int32_t*)stg, pool, leftone, derr);
   }
+  hipLaunchKernelGGL(k_rc_check, dim3(grid_of(NC, kWaves)), dim3(kBlock), 0, s, NC, N, (const int64_t*)colptr,
+                     (const int32_t*)pool, derr);
   RCCHK(hipGetLastError());
   RCCHK(hipMemcpyAsync(&B->host[4], derr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   RCCHK(hipStreamSynchronize(s));
-  if (*(uint32_t*)&B->host[4]) {
-    if (err) *err = "coboundary lists inconsistent with the common-neighbour counts";
+  if (const uint32_t bad = *(uint32_t*)&B->host[4]) {
+    if (err)
+      *err = (bad & 1u) ? "coboundary lists inconsistent with the common-neighbour counts"
+                        : "coboundary column not strictly increasing or out of range";
     return SSQ_EDEVICE;
   }
 
+  auto tb1 = std::chrono::steady_clock::now();
   // ---- reduction ----
   ssq_stats st{};
   bool oom = false;
@@ -1188,6 +1221,9 @@ static int rc_group(RipsCo* B, PrebuiltTarget* eng, hipStream_t s, int c0, int c
   acc->n_dense_additions += st.n_dense_additions;
   acc->dense_bytes += st.dense_bytes;
   acc->t_add_long_kernel_ms += st.t_add_long_kernel_ms;
+  acc->n_huge_additions += st.n_huge_additions;
+  acc->huge_bytes += st.huge_bytes;
+  acc->t_add_huge_kernel_ms += st.t_add_huge_kernel_ms;
   acc->max_chain = std::max(acc->max_chain, st.max_chain);
   acc->max_addition_len = std::max(acc->max_addition_len, st.max_addition_len);
   acc->store_bytes = std::max(acc->store_bytes, st.store_bytes);
@@ -1195,40 +1231,56 @@ static int rc_group(RipsCo* B, PrebuiltTarget* eng, hipStream_t s, int c0, int c
 
   // ---- pairs ----
   int32_t* flag;
-  int64_t *pos, *hv, *he, *ob;
+  int64_t *pos, *hv, *he;
   RCGROW(flag, int32_t, NC, flag);
   RCGROW(pos, int64_t, NC + 1, pos);
   RCGROW(hv, int64_t, G, hv);
   RCGROW(he, int64_t, G, he);
-  RCGROW(ob, int64_t, G + 1, ob);
   hipLaunchKernelGGL(k_rc_pairflag, dim3(grid_of(NC)), dim3(kBlock), 0, s, q, (const int32_t*)ecl, eng->pre_low(),
                      eng->pre_piv(), flag);
   if (int rc = scan64(B, s, (const int32_t*)flag, NC, pos, err)) return rc;
   hipLaunchKernelGGL(k_rc_paircount, dim3(grid_of(G)), dim3(kBlock), 0, s, q, (const int64_t*)pos, hv, he);
-  hipLaunchKernelGGL(k_rc_pairbase, dim3(1), dim3(64), 0, s, G, (const int64_t*)hv, (const int64_t*)he, ob);
   hipLaunchKernelGGL(k_rc_pairemit, dim3(grid_of(NC)), dim3(kBlock), 0, s, q, (const int32_t*)ecl, eng->pre_low(),
-                     (const int32_t*)flag, (const int64_t*)pos, (const int64_t*)hv, (const int64_t*)ob,
-                     (const int32_t*)rlo, (const int32_t*)rhi, (const int64_t*)TB, d_out + 2 * out_base);
+                     (const int32_t*)flag, (const int64_t*)pos, (const int64_t*)hv, d_slot, (const int32_t*)rlo,
+                     (const int32_t*)rhi, (const int64_t*)TB, stage);
   RCCHK(hipGetLastError());
-  std::vector<int64_t> hob((size_t)G + 1);
-  RCCHK(hipMemcpyAsync(hob.data(), ob, sizeof(int64_t) * (G + 1), hipMemcpyDeviceToHost, s));
+  std::vector<int64_t> hh(2 * (size_t)G);
+  RCCHK(hipMemcpyAsync(hh.data(), hv, sizeof(int64_t) * G, hipMemcpyDeviceToHost, s));
+  RCCHK(hipMemcpyAsync(hh.data() + G, he, sizeof(int64_t) * G, hipMemcpyDeviceToHost, s));
   RCCHK(hipStreamSynchronize(s));
-  for (int g = 0; g <= G; g++) pair_off[c0 + g] = out_base + hob[(size_t)g];
+  for (int g = 0; g < G; g++) counts[clouds[g]] = hh[(size_t)g] + hh[(size_t)(G + g)];
+  if (B->trace) {
+    auto tb2 = std::chrono::steady_clock::now();
+    std::fprintf(stderr,
+                 "[ssq] group of %d clouds (first %d): %lld vertices, %lld edges, %lld triangles, %lld tie triangles; "
+                 "build %.1f ms, reduce %.1f ms (init %.1f, rounds %lld, pages %lld, additions %lld, tiled %lld, "
+                 "huge %lld, longest chain %lld, pending %lld), pairs %.1f ms\n",
+                 G, clouds[0], (long long)NV, (long long)NE, (long long)NT, (long long)ntie,
+                 std::chrono::duration<double, std::milli>(tb1 - tb0).count(), st.t_total_ms, st.t_init_ms,
+                 (long long)st.n_rounds, (long long)st.n_pages, (long long)st.n_additions,
+                 (long long)st.n_tiled_additions, (long long)st.n_huge_additions, (long long)st.max_chain,
+                 (long long)st.n_pending,
+                 std::chrono::duration<double, std::milli>(tb2 - tb1).count() - st.t_total_ms);
+    std::fflush(stderr);
+  }
   return SSQ_OK;
 }
 
-int ripsco_reduce(RipsCo* B, PrebuiltTarget* eng, int count, const float* d_pts, const int64_t* pbase, int pdim,
-                  const float* radius, const ssq_params& p, int64_t* d_pairs, int64_t pairs_cap, int64_t* pair_off,
-                  int64_t mem_budget, ssq_stats* st, std::string* err, const int64_t** d_pairs_owned) {
-  if (!B || !eng || count < 1 || !pbase || !radius || pdim < 1 || !pair_off || (pbase[count] > 0 && !d_pts) ||
-      (!d_pairs && !d_pairs_owned))
+int ripsco_reduce(RipsCo* const* Bs, PrebuiltTarget* const* engs, int nstreams, int count, const float* d_pts,
+                  const int64_t* pbase, int pdim, const float* radius, const ssq_params& p, int64_t* d_pairs,
+                  int64_t pairs_cap, int64_t* pair_off, int64_t mem_budget, ssq_stats* st, std::string* err,
+                  const int64_t** d_pairs_owned) {
+  if (!Bs || !engs || nstreams < 1 || count < 1 || !pbase || !radius || pdim < 1 || !pair_off ||
+      (pbase[count] > 0 && !d_pts) || (!d_pairs && !d_pairs_owned))
     return SSQ_EINVAL;
-  hipStream_t s = eng->pre_stream();
+  RipsCo* B = Bs[0];  // holds the sizing pass, the slots and the output
+  hipStream_t s = engs[0]->pre_stream();
   auto T0 = std::chrono::steady_clock::now();
-  ssq_stats acc{};
   if (int rc = rc_size(B, s, count, d_pts, pbase, pdim, radius, err)) return rc;
-  int64_t need = 0;
-  for (int c = 0; c < count; c++) need += (pbase[c + 1] - pbase[c]) + B->h_edges[(size_t)c];
+  // every cloud gets a slot of (vertices + edges) pairs, an upper bound of its pair count
+  std::vector<int64_t> slot((size_t)count + 1, 0);
+  for (int c = 0; c < count; c++) slot[c + 1] = slot[c] + (pbase[c + 1] - pbase[c]) + B->h_edges[(size_t)c];
+  const int64_t need = slot[count];
   if (!d_pairs) {  // the builder's own buffer
     RCGROW(pairs, int64_t, 2 * std::max<int64_t>(need, 1), d_pairs);
     pairs_cap = need;
@@ -1238,56 +1290,127 @@ int ripsco_reduce(RipsCo* B, PrebuiltTarget* eng, int count, const float* d_pts,
     if (err) *err = "pair buffer too small (needs one pair per vertex and edge)";
     return SSQ_ENOMEM;
   }
+  int64_t *stage, *dslot, *doff;
+  RCGROW(stage, int64_t, 2 * std::max<int64_t>(need, 1), stage);
+  RCGROW(slots, int64_t, 2 * ((int64_t)count + 1), dslot);
+  doff = dslot + count + 1;
   if (mem_budget <= 0)  // test hook: a small budget forces many groups
     if (const char* e = std::getenv("SSQ_RIPS_GROUP_BYTES")) mem_budget = std::atoll(e);
   if (mem_budget <= 0) {  // fixed at the first call, so that every call groups alike
     if (B->budget <= 0) {
       size_t fr = 0, tot = 0;
       RCCHK(hipMemGetInfo(&fr, &tot));
-      B->budget = (int64_t)((double)fr * 0.75);
+      B->budget = (int64_t)((double)fr * 0.75) - (int64_t)nstreams * kHugeRegionBytes;
     }
-    mem_budget = B->budget;
+    mem_budget = B->budget / nstreams;  // the streams' groups are in memory together
   }
-  // groups: consecutive clouds while the group's estimated bytes fit the budget and its cells
-  // stay below 2^31
-  int64_t arena_hint = 0;
-  int64_t out_base = 0;
-  pair_off[0] = 0;
-  int c0 = 0;
-  while (c0 < count) {
-    int c1 = c0;
+  // Groups: clouds by decreasing triangle count, consecutive ones while the group's estimated
+  // bytes fit the budget and its cells stay below kGroupCells.  The largest clouds (whose
+  // reductions hold the longest chains of additions) share groups, and the groups run on the
+  // streams side by side.
+  std::vector<int> order((size_t)count);
+  for (int c = 0; c < count; c++) order[(size_t)c] = c;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return B->h_tris[(size_t)x] > B->h_tris[(size_t)y]; });
+  std::vector<std::pair<int, int>> groups;  // [begin, end) in order
+  for (int i0 = 0; i0 < count;) {
+    int i1 = i0;
     int64_t nv = 0, ne = 0, nt = 0, n2 = 0, ew = 0;
-    while (c1 < count) {
-      const int64_t n = pbase[c1 + 1] - pbase[c1], E = B->h_edges[(size_t)c1], T = B->h_tris[(size_t)c1];
+    while (i1 < count) {
+      const int c = order[(size_t)i1];
+      const int64_t n = pbase[c + 1] - pbase[c], E = B->h_edges[(size_t)c], T = B->h_tris[(size_t)c];
       const int64_t W = (n + 63) / 64;
       const double bytes = group_bytes(nv + n, ne + E, nt + T, n2 + n * n, ew + E * W, (nt + T) / 16);
       const bool fits = nv + n + ne + E + nt + T <= kGroupCells && bytes <= (double)mem_budget;
-      if (!fits && c1 > c0) break;
+      if (!fits && i1 > i0) break;
       nv += n;
       ne += E;
       nt += T;
       n2 += n * n;
       ew += E * W;
-      c1++;
+      i1++;
       if (!fits) break;  // one cloud alone: try it anyway
     }
-    int rc;
-    for (int attempt = 0;; attempt++) {
-      rc = rc_group(B, eng, s, c0, c1, d_pts, pdim, p, d_pairs, out_base, pair_off, &arena_hint, &acc, err);
-      if (rc != -1) break;
-      if (attempt > 6) {
-        if (err) *err = "column store exhausted";
-        return SSQ_ENOMEM;
+    groups.emplace_back(i0, i1);
+    i0 = i1;
+  }
+  std::vector<int64_t> counts((size_t)count, 0);
+  std::vector<ssq_stats> accs((size_t)nstreams);
+  std::vector<std::string> errs((size_t)nstreams);
+  std::vector<int> rcs((size_t)nstreams, SSQ_OK);
+  std::atomic<int> next{0};
+  std::atomic<bool> stop{false};
+  auto worker = [&](int k) {
+    if (hipSetDevice(engs[k]->pre_device()) != hipSuccess) {
+      rcs[(size_t)k] = SSQ_EDEVICE;
+      stop = true;
+      return;
+    }
+    int64_t arena_hint = 0;
+    for (int gi; !stop && (gi = next++) < (int)groups.size();) {
+      const int i0 = groups[(size_t)gi].first, i1 = groups[(size_t)gi].second;
+      int rc;
+      for (int attempt = 0;; attempt++) {
+        rc = rc_group(Bs[k], B, engs[k], engs[k]->pre_stream(), order.data() + i0, i1 - i0, d_pts, pdim, p, stage,
+                      slot.data(), counts.data(), &arena_hint, &accs[(size_t)k], &errs[(size_t)k]);
+        if (rc != -1) break;
+        if (attempt > 6) {
+          errs[(size_t)k] = "column store exhausted";
+          rc = SSQ_ENOMEM;
+          break;
+        }
+      }
+      if (rc) {
+        rcs[(size_t)k] = rc;
+        stop = true;
       }
     }
-    if (rc) return rc;
-    out_base = pair_off[c1];
-    c0 = c1;
-  }
+  };
+  std::vector<std::thread> pool;
+  for (int k = 1; k < nstreams; k++) pool.emplace_back(worker, k);
+  worker(0);
+  for (auto& t : pool) t.join();
+  for (int k = 0; k < nstreams; k++)
+    if (rcs[(size_t)k]) {
+      if (err) *err = errs[(size_t)k];
+      return rcs[(size_t)k];
+    }
+  // packed output: cloud c's pairs at [pair_off[c], pair_off[c + 1])
+  pair_off[0] = 0;
+  for (int c = 0; c < count; c++) pair_off[c + 1] = pair_off[c] + counts[(size_t)c];
+  RCCHK(hipMemcpyAsync(dslot, slot.data(), sizeof(int64_t) * (count + 1), hipMemcpyHostToDevice, s));
+  RCCHK(hipMemcpyAsync(doff, pair_off, sizeof(int64_t) * (count + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_rc_pack, dim3((unsigned)std::min(count, 4096)), dim3(kBlock), 0, s, count,
+                     (const int64_t*)dslot, (const int64_t*)doff, (const int64_t*)stage, d_pairs);
+  RCCHK(hipGetLastError());
+  RCCHK(hipStreamSynchronize(s));
   if (st) {
+    ssq_stats acc{};
+    for (const ssq_stats& a : accs) {
+      acc.n_pending += a.n_pending;
+      acc.n_apparent += a.n_apparent;
+      acc.n_pages += a.n_pages;
+      acc.n_rounds += a.n_rounds;
+      acc.n_additions += a.n_additions;
+      acc.add_bytes += a.add_bytes;
+      acc.t_add_kernel_ms += a.t_add_kernel_ms;
+      acc.n_add_launches += a.n_add_launches;
+      acc.n_tiled_additions += a.n_tiled_additions;
+      acc.n_dense_additions += a.n_dense_additions;
+      acc.dense_bytes += a.dense_bytes;
+      acc.t_add_long_kernel_ms += a.t_add_long_kernel_ms;
+      acc.n_huge_additions += a.n_huge_additions;
+      acc.huge_bytes += a.huge_bytes;
+      acc.t_add_huge_kernel_ms += a.t_add_huge_kernel_ms;
+      acc.max_chain = std::max(acc.max_chain, a.max_chain);
+      acc.max_addition_len = std::max(acc.max_addition_len, a.max_addition_len);
+      acc.store_bytes = std::max(acc.store_bytes, a.store_bytes);
+      acc.t_reduce_ms += a.t_reduce_ms;
+    }
     *st = acc;
+    st->n_pages = (int64_t)groups.size();  // groups of this call
     st->t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - T0).count();
-    st->t_init_ms = st->t_total_ms - acc.t_reduce_ms;
+    st->t_init_ms = 0.0;
   }
   return SSQ_OK;
 }

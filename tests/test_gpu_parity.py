@@ -220,10 +220,14 @@ def _circle(n, seed):
 
 @pytest.mark.parametrize("env", [{}, {"SSQ_TEST_SHORT_MAX": "16"},
                                  {"SSQ_TEST_SHORT_MAX": "16", "SSQ_TEST_WINDOW_BITS": "256"},
-                                 {"SSQ_TEST_WINDOW_BITS": "4096"}])
+                                 {"SSQ_TEST_WINDOW_BITS": "4096"},
+                                 {"SSQ_TEST_WINDOW_BITS": "256", "SSQ_TEST_GLOBAL_BITS": "2048"},
+                                 {"SSQ_TEST_SHORT_MAX": "16", "SSQ_TEST_WINDOW_BITS": "64",
+                                  "SSQ_TEST_GLOBAL_BITS": "64"}])
 def test_long_column_paths(env, monkeypatch):
-    """Noisy circles in cohomology grow long cocycle columns: dense LDS bit-set path, the
-    fallback to tiled merges when a pivot leaves the window, and both with forced limits."""
+    """Noisy circles in cohomology grow long cocycle columns: the LDS bit set, the device-memory
+    bit set for columns that leave the LDS window, the tiled merges past that one, each forced
+    by smaller limits."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     hd = S.Handle(0)

@@ -51,9 +51,11 @@ def test_crafted_additions_match_oracle(scale, dual):
         h.close()
 
 
-@pytest.mark.parametrize("env", [{"SSQ_TEST_SHORT_MAX": "16"}, {"SSQ_TEST_SHORT_MAX": "16", "SSQ_TEST_WINDOW_BITS": "64"}])
+@pytest.mark.parametrize("env", [{"SSQ_TEST_SHORT_MAX": "16"}, {"SSQ_TEST_SHORT_MAX": "16", "SSQ_TEST_WINDOW_BITS": "64"},
+                                 {"SSQ_TEST_SHORT_MAX": "16", "SSQ_TEST_WINDOW_BITS": "64", "SSQ_TEST_GLOBAL_BITS": "64"}])
 def test_crafted_additions_long_paths(monkeypatch, env):
-    """The same additions forced through the workgroup bit set and its tiled fallback."""
+    """The same additions forced through the LDS bit set, the device-memory bit set and the
+    tiled merges."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     h = S.Handle(0)

@@ -1,24 +1,25 @@
-"""Benchmark: batched persistent-homology reduction on one MI355X.
+"""Benchmark: batched persistent homology of many Vietoris-Rips complexes on one MI355X.
 
-Workload (the metric's batched stream, sizes scaled to fit a bench run): many independent
-Vietoris-Rips complexes, point counts log-uniform in [nmin, nmax], clouds 1/3 each uniform
-cube, noisy circle (sigma 0.05) and noisy torus (R=1, r=0.3, sigma 0.03) in R^3, each with
-its own threshold at the 30th percentile of its pairwise distances, dimension <= 2, float32
-filtration, int32 indices.  The boundary matrices are resident in HBM (block-diagonal)
-before the timed region; one step reduces all of them to persistence pairs in HBM.
+Headline workload (the batched stream as specified): 2048 independent point clouds, point
+counts log-uniform in [64, 2048], clouds 1/3 each uniform cube, noisy circle (sigma 0.05) and
+noisy torus (R=1, r=0.3, sigma 0.03) in R^3, each with its own threshold at the 30th
+percentile of its pairwise distances, simplices up to dimension 2, float32 filtration, int32
+indices: 1.9e8 edges and 1.9e10 triangles per step.  The points are resident in HBM before the
+timed region; one step turns every cloud's points into its persistence pairs in HBM (sizing,
+Rips coboundary matrices built straight into the engine's layout, cohomology reduction, pairs).
 
-Also measured: the plain per-complex GPU reduction (one engine call per complex, in
-sequence), a served stream (Poisson arrivals, p50/p99 latency), bit-exact parity of a sample
-against the CPU oracle, the CPU oracle's own rate on that sample (cpu_baseline), and the
-column-addition kernel's achieved algorithmic bandwidth (roofline).
+Also measured: bit-exact parity of a sample against the CPU oracle (and structural checks of
+every cloud's pairs), the oracle's own rate on that sample (cpu_baseline), the plain
+per-complex GPU reduction (one engine call per cloud), a served stream (20,000 Poisson
+requests per offered load, p50/p99 latency, every result checked against the oracle), the
+large single-complex configuration C2 (model 3), and the roofline of the dominant kernel.
 
-  python bench.py [--steps K] [--warmup W] [--complexes C] [--nmax 256] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--complexes C] [--nmax 2048] [--no-cpu] [--served N]
 """
 import argparse
 import json
 import os
 import sys
-import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 
@@ -30,6 +31,14 @@ import numpy as np  # noqa: E402
 
 METRIC = "pairs bit-exact; diagrams/s + p50/p99: batched server vs plain per-complex GPU"
 HBM_PEAK_GBS = 8000.0
+BATCH_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=64, level2=1024)
+SERVED_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=16384, level1=64, level2=1024)
+SERVED_MATRIX_PARAMS = dict(dualize=0, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=1000,
+                            level2=16000)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def make_cloud(rng, kind, n):
@@ -50,77 +59,26 @@ def percentile_threshold(X, q):
 
 
 def build_stream(count, nmin, nmax, seed, q=30.0):
-    import specseqplusplus_amd as S
+    """(points float32, threshold) per cloud."""
     rng = np.random.default_rng(seed)
     ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).astype(int)
-    specs = []
-    for i in range(count):
-        X = make_cloud(rng, i % 3, int(ns[i])).astype(np.float32)
-        specs.append((X, percentile_threshold(X, q)))
+    pts = [make_cloud(rng, i % 3, int(ns[i])).astype(np.float32) for i in range(count)]
     with ThreadPoolExecutor(8) as ex:
-        mats = list(ex.map(lambda s: S.vr_build(s[0], s[1], 2), specs))
-    return specs, mats
+        rs = list(ex.map(lambda X: percentile_threshold(X, q), pts))
+    return pts, rs
 
 
-def block_diagonal(mats):
-    base = np.cumsum([0] + [m.n for m in mats])
-    nzb = np.cumsum([0] + [m.nnz for m in mats])
-    colptr = np.empty(base[-1] + 1, np.int64)
-    rows = np.empty(nzb[-1], np.int32)
-    dims = np.empty(base[-1], np.int32)
-    for i, m in enumerate(mats):
-        colptr[base[i]:base[i + 1] + 1] = m.colptr + nzb[i]
-        rows[nzb[i]:nzb[i + 1]] = m.rows + base[i]
-        dims[base[i]:base[i + 1]] = m.dims
-    return colptr, rows, dims, base
-
-
-def split_pairs(pairs, base):
-    out = []
-    cut = np.searchsorted(pairs[:, 0], base)
-    for i in range(len(base) - 1):
-        out.append(pairs[cut[i]:cut[i + 1]] - base[i])
-    return out
-
-
-BATCH_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=64, level2=1024)
-SERVED_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=16384, level1=64, level2=1024)
-SERVED_MATRIX_PARAMS = dict(dualize=0, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=1000,
-                            level2=16000)
-
-
-def build_served_pool(seed, n_clouds=48, n_mats=12):
-    """Requests for the served stream and their reference pairs (computed alone on the GPU
-    once here; the oracle checks a sample of them in the parity leg)."""
-    import specseqplusplus_amd as S
-    from specseqplusplus_amd.alpha import alpha_complex
-    rng = np.random.default_rng(seed)
-    pts = []
-    for i in range(n_clouds):
-        n = int(np.exp(rng.uniform(np.log(64), np.log(4096))))
-        X = make_cloud(rng, i % 3, n).astype(np.float32)
-        q = min(30.0, 100.0 * 20.0 / (n - 1))  # 30th percentile, capped at mean degree 20
-        pts.append((X, percentile_threshold(X, q)))
-    mats = []
-    for i in range(n_mats):
-        cols = np.exp(rng.uniform(np.log(1e4), np.log(1e6)))
-        npts = max(64, int(cols / 28.5))
-        P = rng.random((npts, 3)) * 100.0 * (npts / 1e5) ** (1 / 3)
-        mats.append(alpha_complex(P, 90.0))
-    h = S.Handle(0)
-    p = S.params(**SERVED_PARAMS)
-    ref_pts = [h.reduce(S.vr_build(X, r, 2), p)[0] for X, r in pts]
-    ref_mats = [h.reduce(m, S.params(**SERVED_MATRIX_PARAMS))[0] for m in mats]
-    h.close()
-    return pts, mats, (ref_pts, ref_mats)
-
-
-# Column-addition kernels: (name, time field, algorithmic-bytes field).  Algorithmic bytes of
-# one addition a <- a xor b are 4*(|a| + |b| + |a xor b|) + 24 (read both columns, write the
-# result, plus pivot lookups and column metadata); the engine sums them per kernel.
+# ---------------------------------------------------------------------------------------
+# roofline
+# ---------------------------------------------------------------------------------------
+# Column-addition kernels: (name, time field, algorithmic-bytes field, additions field).  One
+# addition a <- a xor b costs 4*(|a| + |b| + |a xor b|) + 24 algorithmic bytes (read both
+# columns, write the result, plus pivot lookups and column metadata); the engine sums them per
+# kernel.
 ADD_KERNELS = [("k_add", "t_add_kernel_ms", "add_bytes"),
-               ("k_add_long", "t_add_long_kernel_ms", "dense_bytes")]
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batched_rips.txt")
+               ("k_add_long", "t_add_long_kernel_ms", "dense_bytes"),
+               ("k_add_huge", "t_add_huge_kernel_ms", "huge_bytes")]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c4.txt")
 
 
 def measured_traffic(kernel):
@@ -137,58 +95,314 @@ def measured_traffic(kernel):
     return None
 
 
-def roofline_record(stats):
-    """Roofline of the column-addition kernel with the largest measured time: algorithmic bytes
-    per launch over its average launch duration (HIP events on the engine's stream)."""
-    launches = sum(s["n_add_launches"] for s in stats)  # k_add and k_add_long launch once per round each
-    per = []
-    for name, tf, bf in ADD_KERNELS:
-        per.append((sum(s[tf] for s in stats), sum(s[bf] for s in stats), name))
+def roofline_record(st):
+    """Roofline of the column-addition kernel with the largest measured time in one step on one
+    stream: its algorithmic bytes over its summed launch durations (HIP events on the stream it
+    is launched on)."""
+    per = [(st[tf], st[bf], name) for name, tf, bf in ADD_KERNELS]
     ms, nbytes, name = max(per)
+    launches = max(1, int(st["n_add_launches"]))  # each add kernel launches once per round
     achieved = (nbytes / 1e9) / (ms / 1e3) if ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(name),
-            "kernel": name, "launches": int(launches), "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
-            "bytes_per_launch": round(nbytes / max(launches, 1), 1),
-            "kernels_ms": {n: round(t, 3) for t, _, n in per}}
+            "kernel": name, "launches": launches, "avg_launch_us": round(ms * 1e3 / launches, 2),
+            "bytes_per_launch": round(nbytes / launches, 1),
+            "kernels_ms_per_step": {n: round(t, 3) for t, _, n in per}}
 
 
-def single_config(args):
-    """One single-complex configuration: the GPU reduction with the matrix resident in HBM,
-    the oracle on one host core, and B_alg / t_gpu, where B_alg = the oracle's addition bytes
-    (same sub-complex) + 4*nnz + 12*N for the input pass + 16*#pairs for the output."""
+# ---------------------------------------------------------------------------------------
+# legs
+# ---------------------------------------------------------------------------------------
+def batched_leg(h, pts, rs, args):
+    """The headline: every cloud's pairs, points resident in HBM, three streams."""
+    import specseqplusplus_amd as S
+    pbase = np.concatenate([[0], np.cumsum([len(X) for X in pts])]).astype(np.int64)
+    d_pts = h.device_array(np.concatenate(pts).astype(np.float32))
+    rad = np.array(rs, np.float32)
+    edges, tris = h.rips_count(d_pts, pbase, rad)
+    cap = int(pbase[-1] + edges.sum())
+    d_pairs = h.device_empty(2 * cap, np.int64)
+    p = S.params(**BATCH_PARAMS)
+    log(f"workload: {len(pts)} clouds, {pbase[-1]} points, {edges.sum()} edges, {tris.sum()} triangles")
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        h.batch_reduce_points_device(d_pts, pbase, rad, d_pairs, cap, p)
+        log(f"warmup {i}: {time.perf_counter() - t:.2f} s")
+    h.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        off, st = h.batch_reduce_points_device(d_pts, pbase, rad, d_pairs, cap, p, stats=True)
+        stats.append(st)
+    h.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"batched: {dt * 1e3 / args.steps:.1f} ms/step, {len(pts) * args.steps / dt:.1f} diagrams/s")
+    allpairs = d_pairs.numpy(2 * int(off[-1])).reshape(-1, 2)
+    per = [allpairs[off[i]:off[i + 1]] for i in range(len(pts))]
+    # one more step on one stream for the roofline: there, kernel time per step is at most the
+    # step's time
+    os.environ["SSQ_RIPS_STREAMS"] = "1"
+    t = time.perf_counter()
+    _, st1 = h.batch_reduce_points_device(d_pts, pbase, rad, d_pairs, cap, p, stats=True)
+    one_stream_ms = (time.perf_counter() - t) * 1e3
+    del os.environ["SSQ_RIPS_STREAMS"]
+    for a in (d_pts, d_pairs):
+        a.free()
+    return {"ms_per_step": dt * 1e3 / args.steps, "diagrams_per_s": len(pts) * args.steps / dt, "stats": stats,
+            "per": per, "edges": edges, "tris": tris, "pbase": pbase, "one_stream_ms": one_stream_ms,
+            "one_stream_stats": st1}
+
+
+def structural_checks(pts, per, edges, tris):
+    """Every cloud's pairs: births strictly increasing (so sorted), each cell in at most one
+    pair, birth < death < cells, one pair per vertex birth except one per connected component
+    (H0: n - #components pairs), and at most one pair per vertex and edge."""
+    from scipy.sparse.csgraph import connected_components
+    from scipy.spatial.distance import pdist, squareform
+
+    def one(i):
+        X, pr = pts[i], per[i]
+        n = len(X)
+        cells = n + int(edges[i]) + int(tris[i])
+        ok = len(pr) <= n + int(edges[i])
+        if len(pr):
+            b, d = pr[:, 0], pr[:, 1]
+            ok &= bool((np.diff(b) > 0).all() and (b < d).all() and d.max() < cells and b.min() >= 0)
+            ok &= len(np.unique(pr)) == 2 * len(pr)
+        return ok, n, int((pr[:, 0] < n).sum()) if len(pr) else 0
+
+    res = list(ThreadPoolExecutor(16).map(one, range(len(pts))))
+    ok = all(r[0] for r in res)
+    return ok, res
+
+
+def components(X, r):
+    from scipy.sparse.csgraph import connected_components
+    from scipy.spatial.distance import pdist, squareform
+    if len(X) < 2:
+        return len(X)
+    D = squareform(pdist(X.astype(np.float64))).astype(np.float32)  # float32 distances, as built
+    A = D <= np.float32(r)
+    return connected_components(A, directed=False)[0]
+
+
+def parity_leg(h, pts, rs, per, args):
+    """Oracle (cohomology twist reduction of the full Rips matrix) on a sample: the largest
+    cloud and random ones until the time budget is spent, one cloud per host thread.  The
+    matrices are built by the general GPU Rips builder (itself checked cell for cell against
+    the host builder in the tests); the oracle's time is the CPU baseline."""
+    from concurrent.futures import FIRST_COMPLETED, wait
+    from oracle import oracle as O
+    O.build()
+    workers = max(1, min(16, os.cpu_count() or 1))
+    order = [int(np.argmax([len(X) for X in pts]))] + \
+        [int(i) for i in np.random.default_rng(1).permutation(len(pts)) if i != np.argmax([len(X) for X in pts])]
+    it = iter(order)
+    done, busy, exact, cells, oracle_bytes = 0, 0.0, True, 0, 0.0
+    t_start = time.perf_counter()
+
+    def work(i, m):
+        mo = O.Matrix(m.colptr, m.rows, m.dims)
+        tt = time.perf_counter()
+        ref, ost = O.twist_pairs(mo, dual=True, stats=True)
+        return i, ref, time.perf_counter() - tt, m.n, ost["bytes"]
+
+    with ThreadPoolExecutor(workers) as ex:
+        futs = set()
+        while True:
+            while len(futs) < workers and time.perf_counter() - t_start < args.cpu_seconds:
+                i = next(it, None)
+                if i is None:
+                    break
+                futs.add(ex.submit(work, i, h.vr_build_gpu(pts[i], rs[i], 2)))
+            if not futs:
+                break
+            fin, futs = wait(futs, return_when=FIRST_COMPLETED)
+            for f in fin:
+                i, ref, dt, n, ob = f.result()
+                busy += dt
+                done += 1
+                cells += n
+                oracle_bytes += ob
+                exact &= bool(ref.shape == per[i].shape and np.array_equal(ref, per[i]))
+    wall = time.perf_counter() - t_start
+    log(f"parity: {done} clouds ({cells} cells) bit-exact={exact}; oracle {done / busy:.2f}/s per core")
+    cpu = {"value": round(done / wall, 3), "unit": "diagrams/s", "cores": workers, "kind": "port",
+           "single_core_value": round(done / busy, 3),
+           "sample": f"{done} clouds of this stream (the largest, then random ones), {cells} cells; oracle twist "
+                     f"reduction in cohomology (anti-transpose, bit-tree pivot column, clearing), one cloud per "
+                     f"thread on {workers} threads, {wall:.1f} s wall (matrix builds not timed)"}
+    return {"checked": done, "cells": cells, "bit_exact": exact}, cpu
+
+
+def plain_leg(pts, rs, args):
+    """The plain per-complex GPU reduction: one engine call per cloud (the general path: GPU Rips
+    build, anti-transpose, single-complex reduction), on a random sample; and the batched path
+    called with one cloud at a time."""
+    import specseqplusplus_amd as S
+    idx = np.random.default_rng(2).permutation(len(pts))[:args.plain]
+    p = S.params(**BATCH_PARAMS)
+    out = {}
+    for name, env in (("plain_gpu", "1"), ("one_cloud_per_call", None)):
+        if env:
+            os.environ["SSQ_RIPS_GENERAL"] = env
+        h = S.Handle(0)
+        big = int(idx[np.argmax([len(pts[i]) for i in idx])])
+        h.batch_reduce_points([(pts[big], rs[big])], 2, p)  # size the buffers
+        t = time.perf_counter()
+        for i in idx:
+            h.batch_reduce_points([(pts[i], rs[i])], 2, p)
+        out[name] = len(idx) / (time.perf_counter() - t)
+        h.close()
+        os.environ.pop("SSQ_RIPS_GENERAL", None)
+        log(f"{name}: {out[name]:.2f} diagrams/s on {len(idx)} random clouds")
+    return out, len(idx)
+
+
+def served_pool(seed, n_clouds, n_mats):
+    """Requests of the served stream and their reference pairs from the oracle.  Clouds: 64-4096
+    points (log-uniform, the three distributions), threshold the 30th percentile of pairwise
+    distances lowered, for large clouds, to a mean degree of 20; matrices: alpha complexes of
+    1e4-1e6 cells (the alpha generator at C3's density)."""
+    import specseqplusplus_amd as S
+    from oracle import oracle as O
+    from specseqplusplus_amd.alpha import alpha_complex
+    rng = np.random.default_rng(seed)
+    pts = []
+    for i in range(n_clouds):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(4096))))
+        X = make_cloud(rng, i % 3, n).astype(np.float32)
+        q = min(30.0, 100.0 * 20.0 / (n - 1))
+        pts.append((X, percentile_threshold(X, q)))
+    mats = []
+    for i in range(n_mats):
+        cols = np.exp(rng.uniform(np.log(1e4), np.log(1e6)))
+        npts = max(64, int(cols / 28.5))
+        P = rng.random((npts, 3)) * 100.0 * (npts / 1e5) ** (1 / 3)
+        mats.append(alpha_complex(P, 90.0))
+
+    def ref_pts(c):
+        m = S.vr_build(c[0], c[1], 2)
+        return O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims), dual=True)
+
+    def ref_mat(m):
+        return O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims))
+
+    with ThreadPoolExecutor(16) as ex:
+        rp = list(ex.map(ref_pts, pts))
+        rm = list(ex.map(ref_mat, mats))
+    return pts, mats, (rp, rm)
+
+
+def served_leg(args):
+    import specseqplusplus_amd as S
+    from specseqplusplus_amd.server import Server
+    t = time.perf_counter()
+    pool_pts, pool_mats, pool_ref = served_pool(args.seed + 7, args.pool_clouds, args.pool_mats)
+    log(f"served pool: {len(pool_pts)} clouds, {len(pool_mats)} matrices "
+        f"({min(m.n for m in pool_mats)}-{max(m.n for m in pool_mats)} cells), oracle references in "
+        f"{time.perf_counter() - t:.1f} s")
+    served = {"requests_per_load": args.served, "pool": {"clouds": len(pool_pts), "matrices": len(pool_mats)},
+              "reference": "oracle", "loads": []}
+    # plain per-request GPU: the same mix one request at a time (host data in, pairs out)
+    h = S.Handle(0)
+    rng = np.random.default_rng(99)
+    n_plain = max(1, min(args.served, 400))
+    kinds = rng.integers(0, 2, n_plain)
+    picks = rng.integers(0, 1 << 30, n_plain)
+    pp_pts, pp_mat = S.params(**SERVED_PARAMS), S.params(**SERVED_MATRIX_PARAMS)
+    h.reduce(max(pool_mats, key=lambda m: m.n), pp_mat)
+    Xb, rb = max(pool_pts, key=lambda c: len(c[0]))
+    h.batch_reduce_points([(Xb, rb)], 2, pp_pts)
+    t = time.perf_counter()
+    for k in range(n_plain):
+        if kinds[k] == 0:
+            X, r = pool_pts[picks[k] % len(pool_pts)]
+            h.batch_reduce_points([(X, r)], 2, pp_pts)
+        else:
+            h.reduce(pool_mats[picks[k] % len(pool_mats)], pp_mat)
+    served["plain_per_request_diagrams_per_s"] = round(n_plain / (time.perf_counter() - t), 1)
+    h.close()
+    log(f"served mix, plain one request at a time: {served['plain_per_request_diagrams_per_s']} diagrams/s")
+    for qps in args.qps:
+        rng = np.random.default_rng(int(qps))
+        kinds = rng.integers(0, 2, args.served)
+        picks = rng.integers(0, 1 << 30, args.served)
+        gaps = rng.exponential(1.0 / qps, args.served)
+        lat = []
+        with Server(streams=args.server_streams, max_batch=64, max_batch_cells=args.server_cells,
+                    p=S.params(**SERVED_MATRIX_PARAMS), points_p=S.params(**SERVED_PARAMS)) as srv:
+            tickets = []
+            ts = time.perf_counter()
+            nxt = ts
+            for k in range(args.served):
+                nxt += gaps[k]
+                while True:
+                    now = time.perf_counter()
+                    if now >= nxt:
+                        break
+                    time.sleep(min(nxt - now, 0.0005))
+                j = int(picks[k] % (len(pool_pts) if kinds[k] == 0 else len(pool_mats)))
+                if kinds[k] == 0:
+                    X, r = pool_pts[j]
+                    tickets.append((0, j, srv.submit_points(X, r, 2)))
+                else:
+                    tickets.append((1, j, srv.submit_matrix(pool_mats[j])))
+            srv.wait_all()
+            te = time.perf_counter()
+            exact = True
+            for kind, j, tk in tickets:
+                res = srv.poll(tk)
+                lat.append(res[2])
+                exact &= bool(np.array_equal(res[0], pool_ref[kind][j]))
+        lat = np.array(lat)
+        rate = args.served / (te - ts)
+        rec = {"qps_offered": qps, "diagrams_per_s": round(rate, 1),
+               "over_plain": round(rate / served["plain_per_request_diagrams_per_s"], 2),
+               "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "bit_exact_vs_oracle": exact}
+        served["loads"].append(rec)
+        log(f"served {rec}")
+    return served
+
+
+def single_config(args, which=None, quiet=False):
+    """One single-complex configuration: the GPU reduction with the matrix resident in HBM, the
+    oracle on one host core, and B_alg / t_gpu, where B_alg = the oracle's addition bytes (same
+    sub-complex) + 4*nnz + 12*N for the input pass + 16*#pairs for the output."""
     import specseqplusplus_amd as S
     from specseqplusplus_amd import configs as CF
     from oracle import oracle as O
     O.build()
+    which = which or args.single
     t0 = time.time()
-    if args.single == "c1":
+    if which == "c1":
         m, kw, what = S.vr_build(CF.c1_points(), 0.25, 2), dict(CF.C1_PARAMS), "C1: Rips of 512 uniform points, r=0.25, dim<=2"
-    elif args.single == "c2":
+    elif which == "c2":
         m, kw = S.vr_build(CF.c2_points(), 1.0, 2), dict(CF.C2_PARAMS)
         what = "C2: Rips of 4096 Haar rotations in R^9, r=1.0, dim<=2, model 3"
     else:
         from specseqplusplus_amd.alpha import alpha_complex, c3_points
         m, kw = alpha_complex(c3_points(), 90.0), dict(model=0, queue_size=20000, level1=1000, level2=16000)
         what = "C3: alpha complex of 1e5 uniform points in a 100-cube, alpha^2<=90, dim<=3"
-    if args.dualize:
+    if args.dualize and not quiet:
         kw["dualize"] = 1
         what += ", cohomology (dualize)"
-    print(f"[bench] {what}: {m.n} cells, {m.nnz} nonzeros (built in {time.time() - t0:.1f}s)",
-          file=sys.stderr, flush=True)
+    log(f"{what}: {m.n} cells, {m.nnz} nonzeros (built in {time.time() - t0:.1f}s)")
     h = S.Handle(0)
     dm = h.upload(m)
     out = h.device_empty(2 * m.n, np.int64)
     p = S.params(**kw)
+    steps = args.single_steps if quiet else args.steps
     for _ in range(args.warmup):
         h.reduce_resident(dm, out, p)
     h.synchronize()
     sts = []
     t = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         sts.append(h.reduce_resident(dm, out, p))
     h.synchronize()
-    ms = (time.perf_counter() - t) * 1e3 / args.steps
+    ms = (time.perf_counter() - t) * 1e3 / steps
     n_pairs, st = sts[-1]
     pairs = out.numpy(2 * n_pairs).reshape(-1, 2)
     mo = O.Matrix(m.colptr, m.rows, m.dims)
@@ -198,8 +412,8 @@ def single_config(args):
     cpu_s = time.perf_counter() - tt
     b_alg = ost["bytes"] + 4.0 * m.nnz + 12.0 * m.n + 16.0 * len(ref)
     achieved = b_alg / (ms / 1e3) / 1e9
-    rec = {"metric": f"single-complex reduction time ({args.single})", "value": round(ms, 3), "unit": "ms",
-           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+    rec = {"metric": f"single-complex reduction time ({which})", "value": round(ms, 3), "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "dtype": "int32", "data": "synthetic (seeded)",
            "config": {"workload": what, "cells": int(m.n), "nnz": int(m.nnz), "params": kw},
            "bit_exact": bool(np.array_equal(pairs, ref)), "pairs": int(len(ref)),
@@ -211,293 +425,107 @@ def single_config(args):
                             "sample": "the whole matrix, oracle twist reduction (bit-tree pivot column, clearing)"
                                       + (", after the oracle's sequential GUDHI edge collapse (timed)"
                                          if skip is not None else "")}}
-    print(json.dumps(rec))
     h.close()
+    if quiet:
+        return rec
+    print(json.dumps(rec))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--complexes", type=int, default=2048)
     ap.add_argument("--nmin", type=int, default=64)
-    ap.add_argument("--nmax", type=int, default=256)
-    ap.add_argument("--plain", type=int, default=512, help="complexes timed one call each (plain GPU)")
-    ap.add_argument("--served", type=int, default=2000, help="served requests per offered load (0 = skip)")
-    ap.add_argument("--qps", type=float, nargs="+", default=[200.0, 1000.0, 2000.0, 2500.0, 4000.0])
+    ap.add_argument("--nmax", type=int, default=2048)
+    ap.add_argument("--plain", type=int, default=48, help="random clouds timed one call each (plain GPU)")
+    ap.add_argument("--served", type=int, default=20000, help="served requests per offered load (0 = skip)")
+    ap.add_argument("--qps", type=float, nargs="+", default=[1000.0, 2000.0, 2500.0, 4000.0])
+    ap.add_argument("--pool-clouds", type=int, default=192)
+    ap.add_argument("--pool-mats", type=int, default=24)
     ap.add_argument("--server-streams", type=int, default=4)
     ap.add_argument("--server-cells", type=int, default=2_000_000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=45.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c2", action="store_true", help="skip the C2 sub-record")
+    ap.add_argument("--single-steps", type=int, default=5)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--single", choices=["c1", "c2", "c3"],
                     help="time one single-complex configuration instead of the batched stream")
     ap.add_argument("--dualize", action="store_true", help="with --single: reduce in cohomology")
-    ap.add_argument("--streams", type=int, default=3,
-                    help="concurrent groups (one HIP stream each) the batched stream is split into")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="use the default batched parameters instead of tuning each size class")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every complex is independent, so more GPUs would each run "
               "their own copy of this stream; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-
     if args.single:
         single_config(args)
         return
 
     import specseqplusplus_amd as S
-    from specseqplusplus_amd.server import Server
-
-    t0 = time.time()
-    specs, mats = build_stream(args.complexes, args.nmin, args.nmax, args.seed)
-    N, NNZ = sum(m.n for m in mats), sum(m.nnz for m in mats)
-    print(f"[bench] workload: {args.complexes} complexes, {N} cells, {NNZ} nonzeros "
-          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-
-    # size classes (by point count), each tuned on a sample and reduced as one batch
-    from specseqplusplus_amd.autotune import block_diagonal as bdiag, tune
+    t = time.time()
+    pts, rs = build_stream(args.complexes, args.nmin, args.nmax, args.seed)
+    log(f"generated {len(pts)} clouds in {time.time() - t:.1f} s")
     h = S.Handle(0)
-    ns = np.array([len(sp[0]) for sp in specs])
-    split = int(np.sqrt(args.nmin * args.nmax))
-    classes = [np.where(ns < split)[0], np.where(ns >= split)[0]]
-    classes = [c for c in classes if len(c)]
-    # each size class is split into interleaved groups; every group is one block-diagonal
-    # batch on its own handle (HIP stream), and a step reduces all groups concurrently
-    total_cells = sum(m.n for m in mats)
-    class_params = []
-    for ci, idx in enumerate(classes):
-        if args.no_tune:
-            chosen, table = dict(BATCH_PARAMS), []
-        else:
-            chosen, table = tune(h, [mats[i] for i in idx[:min(len(idx), 96)]])
-        class_params.append((chosen, table))
-    # Each handle owns one HIP stream, and HIP puts a stream on one of GPU_MAX_HW_QUEUES hardware
-    # queues (4 by default) when it is created: a stream created while four others are open
-    # shares a queue and serialises with another busy stream, even after the others close
-    # (4 groups: 32.6 ms/step created with 4 streams open, 46.6 ms with 5).  So every leg
-    # closes its handles before the next one opens its own.
+    B = batched_leg(h, pts, rs, args)
     h.close()
-    plans = []
-    for ci, idx in enumerate(classes):
-        chosen, table = class_params[ci]
-        cells = sum(mats[i].n for i in idx)
-        parts = max(1, int(round(args.streams * cells / total_cells)))
-        for g in range(parts):
-            gidx = idx[g::parts]
-            hg = S.Handle(0)
-            bd, cbase = bdiag([mats[i] for i in gidx])
-            plans.append({"idx": gidx, "h": hg, "dm": hg.upload(bd), "out": hg.device_empty(2 * bd.n, np.int64),
-                          "p": S.params(**chosen), "chosen": chosen, "table": table, "base": cbase,
-                          "cells": bd.n, "nnz": bd.nnz, "cls": ci})
-            del bd
-        print(f"[bench] class {ci}: {len(idx)} complexes, {cells} cells in {parts} group(s), tuned {chosen} "
-              f"({[t['ms'] for t in table]} ms)", file=sys.stderr, flush=True)
-    for pl in plans:
-        pl["h"].synchronize()
-    pool = ThreadPoolExecutor(len(plans))
+    N = int(B["pbase"][-1] + B["edges"].sum() + B["tris"].sum())
+    NNZ = int(2 * B["edges"].sum() + 3 * B["tris"].sum())
+    npairs = int(sum(len(p) for p in B["per"]))
 
-    def passes(pl, k):
-        # k passes over one group, back to back on its stream
-        out = [pl["h"].reduce_resident(pl["dm"], pl["out"], pl["p"]) for _ in range(k)]
-        pl["h"].synchronize()
-        return out
+    ok_struct, res = structural_checks(pts, B["per"], B["edges"], B["tris"])
+    with ThreadPoolExecutor(16) as ex:
+        comps = list(ex.map(lambda i: components(pts[i], rs[i]), range(len(pts))))
+    h0_ok = all(res[i][2] == len(pts[i]) - comps[i] for i in range(len(pts)))
+    log(f"structural checks of all {len(pts)} clouds: {ok_struct}; H0 pair counts = n - components: {h0_ok}")
 
-    def step():
-        return [f.result()[0] for f in [pool.submit(passes, pl, 1) for pl in plans]]
-
-    # A step is one pass over every group.  The timed region brackets all K steps (every group
-    # synchronised before and after), and inside it each group runs its K passes without
-    # waiting for the others, so one group's bandwidth-bound init overlaps another's
-    # latency-bound add rounds (29.6 ms/step against 32.4 with the groups in lockstep).
-    for f in [pool.submit(passes, pl, args.warmup) for pl in plans]:
-        f.result()
-    stats, last = [], []  # every call's stats; each group's last call
-    t = time.perf_counter()
-    futs = [pool.submit(passes, pl, args.steps) for pl in plans]
-    for f in futs:
-        res = f.result()
-        stats.extend(st for _, st in res)
-        last.append(res[-1][1])
-    dt = time.perf_counter() - t
-    ms_per_step = dt * 1e3 / args.steps
-    diagrams_per_s = args.complexes * args.steps / dt
-    print(f"[bench] batched: {ms_per_step:.2f} ms/step, {diagrams_per_s:.0f} diagrams/s", file=sys.stderr, flush=True)
-    per = [None] * args.complexes
-    for pl, (n_pairs, _) in zip(plans, step()):
-        pp = pl["out"].numpy(2 * n_pairs).reshape(-1, 2)
-        for k, arr in enumerate(split_pairs(pp, pl["base"])):
-            per[pl["idx"][k]] = arr
-    class_of = np.zeros(args.complexes, int)
-    for pl in plans:
-        class_of[pl["idx"]] = pl["cls"]
-    pool.shutdown()
-    for pl in plans:
-        for a in (pl["dm"].colptr, pl["dm"].rows, pl["dm"].dims, pl["out"]):
-            a.free()
-        pl["h"].close()
-    h = S.Handle(0) if args.plain > 0 or args.served > 0 else None
-
-    # parity sample against the oracle (CPU); also the CPU baseline rate
-    parity = {"checked": 0, "bit_exact": True}
-    cpu = None
+    parity, cpu = ({"checked": 0}, None)
     if not args.no_cpu:
-        from concurrent.futures import FIRST_COMPLETED, wait
-        from oracle import oracle as O
-        O.build()
-        # one complex per core (the oracle is re-entrant and ctypes drops the GIL during the
-        # call), random order, until the time budget is spent
-        workers = max(1, min(16, os.cpu_count() or 1))
-        order = iter(np.random.default_rng(1).permutation(args.complexes))
+        h = S.Handle(0)
+        parity, cpu = parity_leg(h, pts, rs, B["per"], args)
+        h.close()
+    parity["all_clouds_structural"] = bool(ok_struct and h0_ok)
 
-        def work(i):
-            m = mats[i]
-            tt = time.perf_counter()
-            ref = O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims))
-            return i, ref, time.perf_counter() - tt
-
-        done, busy = 0, 0.0
-        t_start = time.perf_counter()
-        with ThreadPoolExecutor(workers) as ex:
-            futs = set()
-            while True:
-                while len(futs) < 2 * workers and time.perf_counter() - t_start < args.cpu_seconds:
-                    i = next(order, None)
-                    if i is None:
-                        break
-                    futs.add(ex.submit(work, i))
-                if not futs:
-                    break
-                fin, futs = wait(futs, return_when=FIRST_COMPLETED)
-                for f in fin:
-                    i, ref, dt = f.result()
-                    busy += dt
-                    done += 1
-                    ok = ref.shape == per[i].shape and np.array_equal(ref, per[i])
-                    parity["bit_exact"] &= bool(ok)
-        wall = time.perf_counter() - t_start
-        parity["checked"] = done
-        cpu = {"value": done / wall, "unit": "diagrams/s", "cores": workers, "kind": "port",
-               "single_core_value": done / busy,
-               "sample": f"{done} complexes of this stream (random order), oracle twist reduction "
-                         f"(bit-tree pivot column, clearing), one complex per thread on {workers} threads, "
-                         f"{wall:.1f}s wall"}
-        print(f"[bench] parity {parity}; cpu {cpu['value']:.1f} diagrams/s on {workers} threads "
-              f"({cpu['single_core_value']:.1f}/s per core)", file=sys.stderr, flush=True)
-
-    # plain per-complex GPU: one engine call per complex, matrices already resident
-    plain = None
+    plain, n_plain = ({}, 0)
     if args.plain > 0:
-        idx = np.random.default_rng(2).permutation(args.complexes)[:args.plain]
-        cls_p = [S.params(**c) for c, _ in class_params]
-        dev_mats = [(h.upload(mats[i]), cls_p[class_of[i]]) for i in idx]
-        d_pairs = h.device_empty(2 * max(mats[i].n for i in idx), np.int64)
-        big = int(np.argmax([mats[i].n for i in idx]))  # size the engine's buffers first
-        h.reduce_resident(dev_mats[big][0], d_pairs, dev_mats[big][1])
-        h.synchronize()
-        tt = time.perf_counter()
-        for dm, pp in dev_mats:
-            h.reduce_resident(dm, d_pairs, pp)
-        h.synchronize()
-        plain = len(dev_mats) / (time.perf_counter() - tt)
-        del dev_mats, d_pairs
-        print(f"[bench] plain per-complex GPU: {plain:.0f} diagrams/s", file=sys.stderr, flush=True)
+        plain, n_plain = plain_leg(pts, rs, args)
+    served = served_leg(args) if args.served > 0 else None
+    c2 = None if args.no_c2 else single_config(args, "c2", quiet=True)
 
-    # served stream: Poisson arrivals; half point clouds (64-4096 points, Rips built by the
-    # server), half prebuilt alpha-complex matrices (1e4-1e6 columns); several offered loads
-    served = None
-    if args.served > 0:
-        served = {"requests_per_load": args.served, "loads": []}
-        pool_pts, pool_mats, pool_ref = build_served_pool(args.seed + 7)
-        print(f"[bench] served pool: {len(pool_pts)} clouds, {len(pool_mats)} matrices "
-              f"({min(m.n for m in pool_mats)}-{max(m.n for m in pool_mats)} cells)", file=sys.stderr, flush=True)
-        # plain per-request GPU: the same request mix, one request at a time (Rips build on the
-        # GPU for clouds, then a single-complex reduction), host data in and pairs out
-        rng = np.random.default_rng(99)
-        n_plain = max(1, min(args.served, 400))
-        kinds = rng.integers(0, 2, n_plain)
-        picks = rng.integers(0, 1 << 30, n_plain)
-        pp_pts, pp_mat = S.params(**SERVED_PARAMS), S.params(**SERVED_MATRIX_PARAMS)
-        # size the engine's buffers with the largest request of each kind first
-        h.reduce(max(pool_mats, key=lambda m: m.n), pp_mat)
-        Xb, rb = max(pool_pts, key=lambda c: len(c[0]))
-        h.reduce(h.vr_build_gpu(Xb, rb, 2), pp_pts)
-        tt = time.perf_counter()
-        for k in range(n_plain):
-            if kinds[k] == 0:
-                X, r = pool_pts[picks[k] % len(pool_pts)]
-                h.reduce(h.vr_build_gpu(X, r, 2), pp_pts)
-            else:
-                h.reduce(pool_mats[picks[k] % len(pool_mats)], pp_mat)
-        served["plain_per_request_diagrams_per_s"] = round(n_plain / (time.perf_counter() - tt), 1)
-        print(f"[bench] served mix, plain one request at a time: "
-              f"{served['plain_per_request_diagrams_per_s']} diagrams/s", file=sys.stderr, flush=True)
-        h.close()  # the server's workers open their own streams
-        for qps in args.qps:
-            rng = np.random.default_rng(int(qps))
-            kinds = rng.integers(0, 2, args.served)
-            picks = rng.integers(0, 1 << 30, args.served)
-            gaps = rng.exponential(1.0 / qps, args.served)
-            lat = []
-            with Server(streams=args.server_streams, max_batch=64, max_batch_cells=args.server_cells,
-                        p=S.params(**SERVED_MATRIX_PARAMS), points_p=S.params(**SERVED_PARAMS)) as srv:
-                tickets = []
-                ts = time.perf_counter()
-                nxt = ts
-                for k in range(args.served):
-                    nxt += gaps[k]
-                    while True:
-                        now = time.perf_counter()
-                        if now >= nxt:
-                            break
-                        time.sleep(min(nxt - now, 0.0005))
-                    if kinds[k] == 0:
-                        X, r = pool_pts[picks[k] % len(pool_pts)]
-                        tickets.append((0, picks[k] % len(pool_pts), srv.submit_points(X, r, 2)))
-                    else:
-                        tickets.append((1, picks[k] % len(pool_mats), srv.submit_matrix(pool_mats[picks[k] % len(pool_mats)])))
-                srv.wait_all()
-                te = time.perf_counter()
-                exact = True
-                for kind, j, tk in tickets:
-                    res = srv.poll(tk)
-                    lat.append(res[2])
-                    exact &= bool(np.array_equal(res[0], pool_ref[kind][j]))
-            lat = np.array(lat)
-            rate = args.served / (te - ts)
-            rec = {"qps_offered": qps, "diagrams_per_s": round(rate, 1),
-                   "over_plain": round(rate / served["plain_per_request_diagrams_per_s"], 2),
-                   "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                   "bit_exact": exact}
-            served["loads"].append(rec)
-            print(f"[bench] served {rec}", file=sys.stderr, flush=True)
-
-    st = {k: sum(x[k] for x in last) for k in last[0] if isinstance(last[0][k], (int, float))}
-    roofline = roofline_record(stats)
+    st1 = B["one_stream_stats"]
+    roof = roofline_record(st1)
+    roof["one_stream_step_ms"] = round(B["one_stream_ms"], 1)
+    # whole-step algorithmic bytes that no schedule can inflate: one pass over the input rows and
+    # cell records and the pairs written; the additions are left out (so this is a lower bound)
+    b_step = 4.0 * NNZ + 12.0 * N + 16.0 * npairs
+    step_gbs = b_step / (B["ms_per_step"] / 1e3) / 1e9
+    last = B["stats"][-1]
     out = {
-        "metric": METRIC, "value": round(diagrams_per_s, 2), "unit": "diagrams/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "metric": METRIC, "value": round(B["diagrams_per_s"], 2), "unit": "diagrams/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(B["ms_per_step"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic (seeded point clouds -> Vietoris-Rips boundary matrices)",
+        "data": "synthetic (seeded point clouds; every step builds the Vietoris-Rips coboundary matrices on the GPU)",
         "config": {"workload": f"batched: {args.complexes} Rips complexes, n log-uniform [{args.nmin},{args.nmax}], "
-                               f"cube/circle/torus, r = 30th percentile, dim<=2, block-diagonal in HBM",
-                   "cells": int(N), "nnz": int(NNZ),
-                   "streams": args.streams,
-                   "groups": [{"size_class": pl["cls"], "complexes": int(len(pl["idx"])), "cells": int(pl["cells"]),
-                               "nnz": int(pl["nnz"]), "params": pl["chosen"],
-                               "tuning_ms": [t["ms"] for t in pl["table"]]} for pl in plans]},
-        "plain_gpu_diagrams_per_s": round(plain, 2) if plain else None,
-        "batched_over_plain": round(diagrams_per_s / plain, 2) if plain else None,
+                               f"cube/circle/torus, r = 30th percentile, dim<=2; points in HBM -> pairs in HBM",
+                   "points": int(B["pbase"][-1]), "edges": int(B["edges"].sum()), "triangles": int(B["tris"].sum()),
+                   "cells": N, "nnz": NNZ, "pairs": npairs, "streams": 3, "groups_per_step": int(last["n_pages"]),
+                   "params": BATCH_PARAMS},
+        "plain_gpu_diagrams_per_s": round(plain["plain_gpu"], 3) if plain else None,
+        "batched_over_plain": round(B["diagrams_per_s"] / plain["plain_gpu"], 2) if plain else None,
+        "one_cloud_per_call_diagrams_per_s": round(plain["one_cloud_per_call"], 3) if plain else None,
+        "plain_sample": n_plain,
         "served": served, "parity": parity,
-        "engine": {"pending": st["n_pending"], "apparent": st["n_apparent"], "cleared_hd": st["n_cleared_hd"],
-                   "additions": st["n_additions"], "rounds": st["n_rounds"], "pages": st["n_pages"],
-                   "tiled_additions": st["n_tiled_additions"],
-                   "max_chain": max(x["max_chain"] for x in last),
-                   "max_addition_len": max(x["max_addition_len"] for x in last),
-                   "t_init_ms": round(st["t_init_ms"], 3), "t_reduce_ms": round(st["t_reduce_ms"], 3)},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "engine": {"pending": last["n_pending"], "apparent": last["n_apparent"], "additions": last["n_additions"],
+                   "huge_additions": last["n_huge_additions"], "rounds": last["n_rounds"],
+                   "max_chain": last["max_chain"], "max_addition_len": last["max_addition_len"],
+                   "reduce_ms_summed_over_streams": round(last["t_reduce_ms"], 1)},
+        "roofline": roof,
+        "step_roofline": {"b_alg_bytes": b_step, "achieved": round(step_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 6),
+                          "note": "4*nnz + 12*cells + 16*pairs per step over the step time (additions excluded)"},
+        "cpu_baseline": cpu,
+        "c2": c2,
     }
     print(json.dumps(out))
 

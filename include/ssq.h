@@ -80,13 +80,15 @@ typedef struct ssq_stats {
   int64_t n_add_launches; /* column-addition kernel launches                            */
   int64_t n_tiled_additions; /* additions merged tile by tile from HBM                   */
   int64_t n_dense_additions; /* additions into a workgroup's LDS bit-set accumulator      */
-  double dense_bytes;     /* algorithmic bytes of the long-column kernel's additions     */
+  double dense_bytes;     /* bytes those additions move: 4|b|+24 each (the pivot's rows; the
+                             running column stays in the LDS bit set)                     */
   double t_add_long_kernel_ms; /* summed duration of the long-column kernel launches     */
   int64_t max_chain;      /* most additions one column made in one round                */
   int64_t max_addition_len; /* largest |a|+|b| of one addition                          */
   int64_t n_huge_additions; /* additions into a device-memory bit set (columns whose rows
                                spread past the LDS window)                                */
-  double huge_bytes;      /* algorithmic bytes of those additions                          */
+  double huge_bytes;      /* bytes those additions move: 12|b|+24 each (the pivot's rows,
+                             and the bit-set words they toggle, read and written)          */
   double t_add_huge_kernel_ms; /* summed duration of the huge-column kernel launches      */
 } ssq_stats;
 

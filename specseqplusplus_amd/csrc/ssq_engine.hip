@@ -939,7 +939,7 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
         const int32_t newl = red[0];
         a2 = red[1];
         n_dense += 1;
-        n_dbytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
+        n_dbytes += 4ull * (unsigned long long)b_len + 24ull;  // the pivot read; the bit set is in LDS
         max_len = max(max_len, (uint32_t)(a_len + b_len));
         chain += 1;
         a_len = r_len;
@@ -1172,7 +1172,7 @@ __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t
         __syncthreads();
         const int32_t newl = red[0];
         n_add += 1;
-        n_bytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
+        n_bytes += 12ull * (unsigned long long)b_len + 24ull;  // the pivot read, its words toggled (read, write)
         max_len = max(max_len, (uint32_t)(a_len + b_len));
         chain += 1;
         a_len = r_len;
@@ -2868,6 +2868,7 @@ struct Handle {
   // further engines and builders of the batched Rips path: its groups run on several streams
   std::vector<std::unique_ptr<Engine>> xeng;
   std::vector<RipsCo*> xrc;
+  int rips_streams = 0;  // 0: SSQ_RIPS_STREAMS or 3
   DevBuf pts;
   explicit Handle(int dev) : eng(dev) {}
   ~Handle() {
@@ -3207,11 +3208,13 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
   return SSQ_OK;
 }
 
-// Streams of the batched Rips path: SSQ_RIPS_STREAMS (1-4), default 3.  HIP places a stream on
-// one of GPU_MAX_HW_QUEUES hardware queues (4 by default), so at most 4.
-static int rips_streams() {
+// Streams of the batched Rips path: the handle's setting (ssq::handle_set_rips_streams), else
+// SSQ_RIPS_STREAMS (1-4), else 3.  HIP places a stream on one of GPU_MAX_HW_QUEUES hardware
+// queues (4 by default), so at most 4.
+static int rips_streams(const Handle* H) {
   int n = 3;
   if (const char* e = std::getenv("SSQ_RIPS_STREAMS")) n = std::atoi(e);
+  if (H->rips_streams > 0) n = H->rips_streams;
   return std::max(1, std::min(4, n));
 }
 
@@ -3219,7 +3222,7 @@ static int rips_reduce(Handle* H, int count, const float* d_pts, const int64_t* 
                        const ssq_params& p, int64_t* d_pairs, int64_t pairs_cap, int64_t* pair_off, ssq_stats* stats,
                        const int64_t** d_pairs_owned) {
   if (!H->rc && !(H->rc = ripsco_create())) return SSQ_ENOMEM;
-  const int ns = rips_streams();
+  const int ns = rips_streams(H);
   while ((int)H->xeng.size() < ns - 1) {
     std::unique_ptr<Engine> e(new (std::nothrow) Engine(H->eng.device()));
     RipsCo* b = ripsco_create();
@@ -3398,3 +3401,9 @@ extern "C" int ssq_batch_reduce_points_device(ssq_handle* h, int count, const fl
   if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
   return rips_reduce(H, count, d_pts, pbase, pdim, radius, *p, d_pairs, pairs_cap, pair_off, stats, nullptr);
 }
+
+namespace ssq {
+void handle_set_rips_streams(ssq_handle* h, int n) {
+  if (h) h->h->rips_streams = n;
+}
+}  // namespace ssq

@@ -191,6 +191,44 @@ int vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim, ssq_
 
 }  // namespace ssq
 
+namespace ssq {
+
+// Edges and triangles of the Rips complex at threshold r (same distances as vr_build), by
+// adjacency bitsets: triangles are the common neighbours of each edge, counted three times.
+// Used to admit point-cloud requests by their real size.
+void rips_count_host(const float* pts, int64_t npts, int pdim, float r, int64_t* edges, int64_t* triangles) {
+  const int64_t n = npts, W = (n + 63) / 64;
+  std::vector<uint64_t> A((size_t)(n * W), 0);
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = i + 1; j < n; j++) {
+      double acc = 0.0;
+      for (int k = 0; k < pdim; k++) {
+        const double d = (double)pts[i * pdim + k] - (double)pts[j * pdim + k];
+        acc += d * d;
+      }
+      if ((float)std::sqrt(acc) <= r) {
+        A[(size_t)(i * W + (j >> 6))] |= 1ull << (j & 63);
+        A[(size_t)(j * W + (i >> 6))] |= 1ull << (i & 63);
+      }
+    }
+  int64_t e = 0, t3 = 0;
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t w = i >> 6; w < W; w++) {
+      uint64_t m = A[(size_t)(i * W + w)];
+      if (w == (i >> 6)) m &= ~((2ull << (i & 63)) - 1);  // neighbours above i
+      while (m) {
+        const int64_t j = w * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        e++;
+        for (int64_t x = 0; x < W; x++) t3 += __builtin_popcountll(A[(size_t)(i * W + x)] & A[(size_t)(j * W + x)]);
+      }
+    }
+  *edges = e;
+  *triangles = t3 / 3;
+}
+
+}  // namespace ssq
+
 extern "C" int ssq_vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim, ssq_matrix* out) {
   if (!out) return SSQ_EINVAL;
   return ssq::vr_build(pts, npts, pdim, r, maxdim, out);

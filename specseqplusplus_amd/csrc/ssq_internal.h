@@ -19,6 +19,8 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
 
 // Vietoris-Rips boundary matrix (host builder).
 int vr_build(const float* pts, int64_t npts, int pdim, float r, int maxdim, ssq_matrix* out);
+// Its edge and triangle counts (adjacency bitsets; no cells are built).
+void rips_count_host(const float* pts, int64_t npts, int pdim, float r, int64_t* edges, int64_t* triangles);
 
 // GPU Vietoris-Rips builder (dimension <= 2).  Device outputs stay valid until the next
 // build on the same builder.
@@ -73,6 +75,10 @@ int ripsco_reduce(RipsCo* const* b, PrebuiltTarget* const* eng, int nstreams, in
                   const int64_t* pbase, int pdim, const float* radius, const ssq_params& p, int64_t* d_pairs,
                   int64_t pairs_cap, int64_t* pair_off, int64_t mem_budget, ssq_stats* st, std::string* err,
                   const int64_t** d_pairs_owned);
+// Streams the batched Rips path of this handle uses (a server worker uses one: its workers
+// already run side by side).
+void handle_set_rips_streams(ssq_handle* h, int n);
+
 // Triangle count of each cloud from the last ripsco_count / ripsco_reduce call.
 const int64_t* ripsco_triangles(const RipsCo* b);
 const int64_t* ripsco_edges(const RipsCo* b);

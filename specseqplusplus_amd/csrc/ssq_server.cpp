@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -51,10 +52,27 @@ int64_t estimate_bytes(int64_t n, int64_t nnz) {
   return 24 * nnz + 64 * n + (1 << 20);
 }
 
-int64_t estimate_points(int64_t npts, int maxdim) {
-  // unknown clique counts until the matrix is built: budget a dense-ish complex of degree 32
-  const int64_t e = npts * 16, t = maxdim >= 2 ? npts * 160 : 0;
-  return estimate_bytes(npts + e + t, 2 * e + 3 * t);
+// A point cloud's cells, counted exactly on the host at submission (adjacency bitsets: the
+// edges within r, the triangles as common neighbours), and its device bytes as a matrix of
+// that size.  Dimension 3 is bounded by the tetrahedra of the triangle count's cube root.
+void estimate_points(const float* pts, int64_t npts, int pdim, float r, int maxdim, int64_t* cells, int64_t* bytes) {
+  int64_t e = 0, t = 0;
+  if (maxdim >= 1 && npts > 1) ssq::rips_count_host(pts, npts, pdim, r, &e, &t);
+  if (maxdim < 2) t = 0;
+  const int64_t tets = maxdim >= 3 ? t * (int64_t)std::cbrt((double)std::max<int64_t>(t, 1)) : 0;
+  *cells = npts + e + t + tets;
+  *bytes = estimate_bytes(*cells, 2 * e + 3 * t + 4 * tets);
+}
+
+// Host checks of a submitted matrix (rows are range-checked on the device): offsets start at
+// 0, never decrease and end at nnz.
+bool matrix_ok(const ssq_matrix* m) {
+  if (m->n < 0 || m->nnz < 0 || (m->n && (!m->col_ptr || !m->dims)) || (m->nnz && !m->rows)) return false;
+  if (m->n == 0) return m->nnz == 0 || !m->col_ptr || m->col_ptr[0] == 0;
+  if (m->col_ptr[0] != 0 || m->col_ptr[m->n] != m->nnz) return false;
+  for (int64_t c = 0; c < m->n; c++)
+    if (m->col_ptr[c + 1] < m->col_ptr[c]) return false;
+  return true;
 }
 
 // Triangulated k x k grid (vertices, then edges, then triangles: a valid filtration) with
@@ -136,6 +154,9 @@ struct ssq_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Request*> queue[2];  // [0] matrices, [1] point clouds
+  std::deque<Request*> sizing;    // point clouds whose cells are being counted
+  std::condition_variable cv_size;
+  std::vector<std::thread> sizers;
   std::unordered_map<int64_t, Result> done;
   std::vector<std::thread> workers;
   int64_t next_id = 1;
@@ -152,6 +173,7 @@ struct ssq_server {
       std::lock_guard<std::mutex> g(mu);
       init_status = rc;
     } else {
+      ssq::handle_set_rips_streams(h, 1);
       warm_up(h, cfg);
     }
     {
@@ -164,8 +186,8 @@ struct ssq_server {
       int batch_kind = 0;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stop || !queue[0].empty() || !queue[1].empty(); });
-        if (stop && queue[0].empty() && queue[1].empty()) break;
+        cv_work.wait(lk, [&] { return (stop && sizing.empty()) || !queue[0].empty() || !queue[1].empty(); });
+        if (queue[0].empty() && queue[1].empty()) break;  // stopping, nothing left to size
         // one kind per batch (each has its own parameters): the queue whose head is older
         int kind = queue[0].empty() ? 1
                    : queue[1].empty() ? 0
@@ -236,6 +258,33 @@ struct ssq_server {
         const ssq_params* bp = batch_kind ? &cfg.points_params : &cfg.params;
         brc = h ? ssq_batch_reduce(h, ptrs.data(), (int)ptrs.size(), bp, outs.data(), nullptr) : SSQ_EDEVICE;
       }
+      if (brc && h && batch.size() > 1) {
+        // a failed batch is reduced again request by request, so that only a bad request
+        // reports the error
+        brc = SSQ_OK;
+        outs.assign(gpu_points ? batch.size() : ptrs.size(), ssq_pairs{});
+        for (size_t k = 0; k < outs.size(); k++) {
+          const size_t i = gpu_points ? k : idx[k];
+          Request* q = batch[i];
+          int rc1;
+          if (gpu_points) {
+            const float* p1 = q->pts.data();
+            rc1 = ssq_batch_reduce_points(h, 1, &p1, &q->npts, q->pdim, &q->r, q->maxdim, &cfg.points_params,
+                                          &outs[k], nullptr);
+          } else {
+            const ssq_params* bp = batch_kind ? &cfg.points_params : &cfg.params;
+            rc1 = ssq_batch_reduce(h, &ptrs[k], 1, bp, &outs[k], nullptr);
+          }
+          if (rc1) {
+            st[i] = rc1;
+            outs[k] = ssq_pairs{};
+          }
+        }
+        if (gpu_points) {  // results of the retried clouds, by request
+          idx.clear();
+          for (size_t i = 0; i < batch.size(); i++) idx.push_back(i);
+        }
+      }
       const auto t_end = Clock::now();
       {
         std::lock_guard<std::mutex> g(mu);
@@ -250,6 +299,7 @@ struct ssq_server {
             R.status = brc;
             continue;
           }
+          if (R.status) continue;  // failed alone
           R.pairs.assign(outs[k].pairs, outs[k].pairs + 2 * outs[k].np);
           R.ess.assign(outs[k].essential, outs[k].essential + outs[k].ne);
           ssq_pairs_free(&outs[k]);
@@ -269,6 +319,9 @@ struct ssq_server {
     (void)wid;
   }
 
+  // Point clouds are counted (edges, triangles) by sizer threads before they are queued, so
+  // that a large cloud's count never holds up the submitting thread; the latency clock starts
+  // at submission.
   int submit(Request* q, int64_t* ticket) {
     {
       std::lock_guard<std::mutex> g(mu);
@@ -278,12 +331,31 @@ struct ssq_server {
       }
       q->id = next_id++;
       q->t_submit = Clock::now();
-      queue[q->is_points ? 1 : 0].push_back(q);
+      (q->is_points ? sizing : queue[0]).push_back(q);
       outstanding++;
       *ticket = q->id;
     }
-    cv_work.notify_one();
+    (q->is_points ? cv_size : cv_work).notify_one();
     return SSQ_OK;
+  }
+
+  void sizer_main() {
+    for (;;) {
+      Request* q;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_size.wait(lk, [&] { return stop || !sizing.empty(); });
+        if (sizing.empty()) break;  // stopping
+        q = sizing.front();
+        sizing.pop_front();
+      }
+      estimate_points(q->pts.data(), q->npts, q->pdim, q->r, q->maxdim, &q->est_cells, &q->est_bytes);
+      {
+        std::lock_guard<std::mutex> g(mu);
+        queue[1].push_back(q);
+      }
+      cv_work.notify_one();
+    }
   }
 };
 
@@ -302,6 +374,8 @@ extern "C" int ssq_server_create(const ssq_server_config* cfg, ssq_server** out)
     s->cfg.mem_budget_bytes = (int64_t)(fr / 2);
   }
   for (int w = 0; w < cfg->streams; w++) s->workers.emplace_back(&ssq_server::worker_main, s, w);
+  const int nsizers = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
+  for (int k = 0; k < nsizers; k++) s->sizers.emplace_back(&ssq_server::sizer_main, s);
   {  // requests are accepted once every worker has sized its buffers
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] { return s->ready == cfg->streams; });
@@ -311,9 +385,13 @@ extern "C" int ssq_server_create(const ssq_server_config* cfg, ssq_server** out)
 }
 
 extern "C" int ssq_server_submit_matrix(ssq_server* s, const ssq_matrix* m, int64_t* ticket) {
-  if (!s || !m || !ticket || m->n < 0 || (m->n && (!m->col_ptr || !m->dims))) return SSQ_EINVAL;
+  if (!s || !m || !ticket) return SSQ_EINVAL;
   Request* q = new (std::nothrow) Request();
   if (!q) return SSQ_ENOMEM;
+  if (!matrix_ok(m)) {
+    delete q;
+    return SSQ_EFORMAT;
+  }
   q->colptr.assign(m->col_ptr, m->col_ptr + m->n + 1);
   if (m->nnz) q->rows.assign(m->rows, m->rows + m->nnz);
   q->dims.assign(m->dims, m->dims + m->n);
@@ -333,8 +411,6 @@ extern "C" int ssq_server_submit_points(ssq_server* s, const float* pts, int64_t
   q->pdim = pdim;
   q->r = r;
   q->maxdim = maxdim;
-  q->est_bytes = estimate_points(npts, maxdim);
-  q->est_cells = npts * 16 * (maxdim >= 2 ? 10 : 1);
   return s->submit(q, ticket);
 }
 
@@ -374,6 +450,8 @@ extern "C" int ssq_server_destroy(ssq_server* s) {
     std::lock_guard<std::mutex> g(s->mu);
     s->stop = true;
   }
+  s->cv_size.notify_all();
+  for (auto& t : s->sizers) t.join();  // they drain the sizing queue first
   s->cv_work.notify_all();
   s->cv_done.notify_all();
   for (auto& t : s->workers) t.join();

@@ -62,3 +62,49 @@ def test_server_mixed_dimensions_and_parameters():
             m = S.vr_build(a, r, md) if kind == "pts" else a
             pairs, ess, lat = srv.poll(t)
             assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims))), (kind, md)
+
+
+def test_server_bad_matrix_fails_alone():
+    """A malformed matrix among good requests: the host checks reject it at submission (bad
+    offsets), or the device check fails its batch (row out of range), which is then reduced
+    request by request, so only the bad request reports the error; the others equal the
+    oracle."""
+    rng = np.random.default_rng(9)
+    good = [S.vr_build(rng.random((30 + 3 * i, 3)).astype(np.float32), 0.35, 2) for i in range(8)]
+    bad = S.Matrix(good[0].colptr.copy(), good[0].rows.copy(), good[0].dims.copy())
+    bad.rows[-1] = bad.n + 5  # row past the end: caught on the device
+    with Server(streams=1, max_batch=16) as srv:
+        offs = S.Matrix(good[1].colptr.copy(), good[1].rows, good[1].dims)
+        offs.colptr[-1] += 1  # offsets end past nnz: rejected at submission
+        with pytest.raises(L.SSQError):
+            srv.submit_matrix(offs)
+        tickets = [srv.submit_matrix(m) for m in good[:4]] + [srv.submit_matrix(bad)] + \
+                  [srv.submit_matrix(m) for m in good[4:]]
+        srv.wait_all()
+        results = []
+        for t in tickets:
+            try:
+                results.append(srv.poll(t)[0])
+            except L.SSQError as e:
+                results.append(e)
+    assert isinstance(results[4], L.SSQError) and results[4].status == L.SSQ_EFORMAT
+    for m, r in zip(good, results[:4] + results[5:]):
+        assert np.array_equal(r, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))
+
+
+def test_server_dense_cloud_under_a_small_budget():
+    """A dense cloud (about 50 neighbours per point) among sparse ones with a device budget far
+    below its size: admission counts its edges and triangles exactly at submission, it runs on
+    its own, and every result equals the oracle's."""
+    rng = np.random.default_rng(10)
+    reqs = [(rng.random((40 + i, 3)).astype(np.float32), 0.2) for i in range(6)]
+    Xd = rng.random((400, 3)).astype(np.float32)
+    reqs.insert(3, (Xd, 0.35))
+    with Server(streams=2, max_batch=8, mem_budget_bytes=1 << 20, p=S.params(),
+                points_p=S.params(dualize=1, ss_block_size=1 << 30)) as srv:
+        tickets = [srv.submit_points(X, r, 2) for X, r in reqs]
+        srv.wait_all()
+        for (X, r), t in zip(reqs, tickets):
+            m = S.vr_build(X, r, 2)
+            pairs = srv.poll(t)[0]
+            assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims), dual=True)), len(X)

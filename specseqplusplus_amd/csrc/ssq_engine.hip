@@ -199,6 +199,28 @@ __device__ bool wave_alloc(const Dev& d, int32_t need, int64_t* off, int32_t* ca
 // ------------------------------------------------------------------------------------
 // Column state over [0, ncols); with init_rows (ncols == n) also the per-row tables, which a
 // prebuilt matrix brings initialised instead.
+// Byte fill and byte copy as kernels.  The engine fills and copies device memory only through
+// these, never through the runtime's memset and device-to-device copy calls: a caller may
+// capture the engine's work in a graph, where memset nodes have given wrong results on replay.
+// [0, head) and the tail go byte by byte; 16-byte words in between (head aligns dst to 16 bytes;
+// a copy uses words only where src has the same alignment, else words16 = 0).
+__global__ void k_fill_bytes(uint8_t* p, uint32_t v4, int64_t head, int64_t words16, int64_t bytes) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+  if (t < head) p[t] = (uint8_t)v4;
+  const uint4 w = make_uint4(v4, v4, v4, v4);
+  uint4* q = (uint4*)(p + head);
+  for (int64_t i = t; i < words16; i += step) q[i] = w;
+  for (int64_t i = head + 16 * words16 + t; i < bytes; i += step) p[i] = (uint8_t)v4;
+}
+__global__ void k_copy_bytes(uint8_t* dst, const uint8_t* src, int64_t head, int64_t words16, int64_t bytes) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+  if (t < head) dst[t] = src[t];
+  uint4* q = (uint4*)(dst + head);
+  const uint4* r = (const uint4*)(src + head);
+  for (int64_t i = t; i < words16; i += step) q[i] = r[i];
+  for (int64_t i = head + 16 * words16 + t; i < bytes; i += step) dst[i] = src[i];
+}
+
 __global__ void k_setup(Dev d, int init_rows) {
   GRID_STRIDE(c, d.ncols) {
     const int64_t o = d.colptr[c];
@@ -1890,6 +1912,11 @@ __global__ void k_vrank_ends(int64_t ne, const int32_t* vrank, int32_t* eu, int3
 // ------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------
+#define DCHK(x)                \
+  do {                         \
+    const int rc_ = (x);       \
+    if (rc_) return rc_;       \
+  } while (0)
 #define HIPCHK(x)                                                                  \
   do {                                                                             \
     hipError_t e_ = (x);                                                           \
@@ -1924,6 +1951,7 @@ class Engine : public PrebuiltTarget {
     HIPCHK(hipHostMalloc(&host_ctr_, cw::kWords * sizeof(uint32_t), hipHostMallocDefault));
     HIPCHK(hipHostMalloc(&host_tot_, kTotSlots * sizeof(int64_t), hipHostMallocDefault));
     HIPCHK(hipMalloc(&tot_, kTotSlots * sizeof(int64_t)));
+    HIPCHK(hipMalloc(&ctr_, cw::kWords * sizeof(uint32_t)));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_));
     cus_ = prop.multiProcessorCount;
@@ -1955,7 +1983,6 @@ class Engine : public PrebuiltTarget {
   int reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
                     const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
                     ssq_stats* st, int64_t* d_essential = nullptr, int64_t* ne_out = nullptr);
-
   // Prebuilt matrices: a builder writes the rows and the init tables straight into the engine's
   // buffers, then runs the init passes and the page loop on them.  Cells [0, n) are rows;
   // cells [0, ncols) also have columns (colptr[ncols + 1] into the pool, dims[ncols]; both stay
@@ -2014,13 +2041,14 @@ class Engine : public PrebuiltTarget {
   }
   void release() {
     for (DevBuf* b : {&b_pool_, &b_off_, &b_len_, &b_cap_, &b_low_, &b_state_, &b_removed_, &b_flag_, &b_piv_,
-                      &b_left_, &b_work_, &b_ctr_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
+                      &b_left_, &b_work_, &b_misc_, &b_U_, &b_keys_, &b_cnt_, &b_e_, &b_vs_, &b_adj_,
                       &b_hash_, &b_hd_, &b_dcol_, &b_drow_, &b_ddim_, &b_drem_, &b_scan_, &b_pmeta_, &b_prem_,
                       &b_scrc_, &b_kbase_, &b_skey_, &b_stmp_, &b_gbits_})
       if (b->p) (void)hipFree(b->p), b->p = nullptr, b->bytes = 0;
     if (host_ctr_) (void)hipHostFree(host_ctr_), host_ctr_ = nullptr;
     if (host_tot_) (void)hipHostFree(host_tot_), host_tot_ = nullptr;
     if (tot_) (void)hipFree(tot_), tot_ = nullptr;
+    if (ctr_) (void)hipFree(ctr_), ctr_ = nullptr;
     for (auto& e : ev_)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
@@ -2042,6 +2070,34 @@ class Engine : public PrebuiltTarget {
   template <typename KeyF, typename EmitF>
   int compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_host, int slot = kTotScratch);
   // read counters and all total slots in one round trip
+  // dst[0, bytes) = the byte value, on the engine's stream (k_fill_bytes)
+  int dfill(void* dst, int value, size_t bytes) {
+    if (bytes == 0) return SSQ_OK;
+    const uintptr_t a = (uintptr_t)dst;
+    const int64_t head = (int64_t)std::min<size_t>(bytes, (16 - (a & 15)) & 15);
+    const int64_t words = (int64_t)((bytes - (size_t)head) / 16);
+    const uint32_t b = (uint32_t)(value & 0xff);
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>((std::max<int64_t>(words, 16) + 255) / 256, (int64_t)cus_ * 16));
+    hipLaunchKernelGGL(k_fill_bytes, dim3((unsigned)g), dim3(256), 0, stream_, (uint8_t*)dst, b * 0x01010101u, head,
+                       words, (int64_t)bytes);
+    HIPCHK(hipGetLastError());
+    return SSQ_OK;
+  }
+  // device-to-device copy of bytes on the engine's stream (k_copy_bytes); the ranges must not overlap
+  int dcopy(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return SSQ_OK;
+    const uintptr_t a = (uintptr_t)dst, b = (uintptr_t)src;
+    int64_t head = 0, words = 0;
+    if ((a & 15) == (b & 15)) {
+      head = (int64_t)std::min<size_t>(bytes, (16 - (a & 15)) & 15);
+      words = (int64_t)((bytes - (size_t)head) / 16);
+    }
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>((std::max<int64_t>(words, 16) + 255) / 256, (int64_t)cus_ * 16));
+    hipLaunchKernelGGL(k_copy_bytes, dim3((unsigned)g), dim3(256), 0, stream_, (uint8_t*)dst, (const uint8_t*)src, head,
+                       words, (int64_t)bytes);
+    HIPCHK(hipGetLastError());
+    return SSQ_OK;
+  }
   int sync_all() {
     HIPCHK(hipMemcpyAsync(host_ctr_, ctr_dev_, sizeof(uint32_t) * cw::kWords, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipMemcpyAsync(host_tot_, tot_, sizeof(int64_t) * kTotSlots, hipMemcpyDeviceToHost, stream_));
@@ -2068,7 +2124,7 @@ class Engine : public PrebuiltTarget {
       uint32_t* g;
       int rc = ensure(b_gbits_, words, &g);
       if (rc) return rc;
-      HIPCHK(hipMemsetAsync(g, 0, words * sizeof(uint32_t), stream_));
+      DCHK(dfill(g, 0, words * sizeof(uint32_t)));
       gbits_ready_ = true;
     }
     d->gbits = (uint32_t*)b_gbits_.p;
@@ -2092,6 +2148,7 @@ class Engine : public PrebuiltTarget {
   static constexpr int kTotScratch = 16;  // [16..23] counts only needed right away
   static constexpr int kTotSlots = 24;
   int64_t* tot_ = nullptr;       // device
+  uint32_t* ctr_ = nullptr;      // device: the per-call counter block (cw::kWords words)
   int64_t* host_tot_ = nullptr;  // pinned
   uint32_t* ctr_dev_ = nullptr;
 
@@ -2102,7 +2159,7 @@ class Engine : public PrebuiltTarget {
   uint32_t* host_ctr_ = nullptr;
   std::string err_;
   DevBuf b_pool_, b_off_, b_len_, b_cap_, b_low_, b_state_, b_removed_, b_flag_, b_piv_, b_left_, b_work_,
-      b_ctr_, b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
+      b_misc_, b_U_, b_keys_, b_cnt_, b_e_, b_vs_, b_adj_, b_hash_, b_hd_, b_dcol_, b_drow_, b_ddim_,
       b_drem_, b_scan_, b_pmeta_, b_prem_, b_scrc_, b_kbase_, b_skey_, b_stmp_;
   int64_t arena_hint_ = 0;
   bool no_sort_dual_ = false;  // test hook: always use the atomic anti-transpose
@@ -2127,7 +2184,7 @@ int Engine::compact(int64_t n, int nkeys, KeyF key, EmitF emit, int64_t* totals_
     hipLaunchKernelGGL(k_key_count<KeyF>, dim3((unsigned)nb), dim3(kCompactBlock), 0, stream_, n, key, counts,
                        nkeys);
   } else {
-    HIPCHK(hipMemsetAsync(counts, 0, sizeof(uint32_t) * nkeys, stream_));
+    DCHK(dfill(counts, 0, sizeof(uint32_t) * nkeys));
   }
   if ((rc = scan_counts((const int32_t*)counts, nb * nkeys, bases))) return rc;
   hipLaunchKernelGGL(k_key_totals, dim3(1), dim3(64), 0, stream_, nkeys, nb, (const int64_t*)bases, totals);
@@ -2219,7 +2276,7 @@ int Engine::collapse(Dev& d, int64_t n, const int32_t* d_dims, const int64_t* d_
   if (ne == 0) return SSQ_OK;
   int32_t* eu = ecols + n;
   int32_t* ev = ecols + 2 * n;
-  HIPCHK(hipMemsetAsync(d.ctr + cw::kFlags, 0, sizeof(uint32_t), stream_));
+  DCHK(dfill(d.ctr + cw::kFlags, 0, sizeof(uint32_t)));
   hipLaunchKernelGGL(k_edge_ends, dim3(grid_for(ne)), dim3(256), 0, stream_, ne, ecols, d_colptr, d_rows, eu, ev,
                      d.ctr + cw::kFlags);
   std::vector<int32_t> hu((size_t)ne), hv((size_t)ne);
@@ -2263,20 +2320,20 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
   uint8_t* hd;
   int rc = ensure(b_hd_, (size_t)n, &hd);
   if (rc) return rc;
-  HIPCHK(hipMemsetAsync(hd, 0, (size_t)n, stream_));
+  DCHK(dfill(hd, 0, (size_t)n));
   // vertex ranks
   int32_t* vs;
   rc = ensure(b_vs_, (size_t)(n * (kMaxVerts + 1) + 16), &vs);
   if (rc) return rc;
   int32_t* vrank = vs + n * kMaxVerts;
-  HIPCHK(hipMemsetAsync(vrank, 0xff, sizeof(int32_t) * n, stream_));
+  DCHK(dfill(vrank, 0xff, sizeof(int32_t) * n));
   int64_t nv = 0;
   rc = compact(n, 1, KeyVertex{d_dims}, EmitRank{vrank}, &nv);
   if (rc) return rc;
   int bits = 1;
   while (bits < 31 && (1ll << bits) < std::max<int64_t>(nv, 2)) bits++;
   if ((int64_t)bits * (maxdim + 1) > 128) return SSQ_OK;  // keys would not fit 128 bits
-  HIPCHK(hipMemsetAsync(d.ctr + cw::kFlags, 0, sizeof(uint32_t) * 2, stream_));
+  DCHK(dfill(d.ctr + cw::kFlags, 0, sizeof(uint32_t) * 2));
   for (int32_t dd = 0; dd <= maxdim; dd++)
     hipLaunchKernelGGL(k_vertex_sets, dim3(grid_for(n)), dim3(256), 0, stream_, n, dd, d_dims, d_colptr, d_rows,
                        vrank, vs, d.ctr + cw::kFlags);
@@ -2301,7 +2358,7 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
   if (rc) return rc;
   unsigned long long* fill = (unsigned long long*)(aoff + nv + 1);
   int32_t* adj = (int32_t*)(aoff + 2 * (nv + 1));
-  HIPCHK(hipMemsetAsync(aoff, 0, sizeof(int64_t) * 2 * (nv + 1), stream_));
+  DCHK(dfill(aoff, 0, sizeof(int64_t) * 2 * (nv + 1)));
   hipLaunchKernelGGL(k_adj_count, dim3(grid_for(std::max<int64_t>(ne, 1))), dim3(256), 0, stream_, ne, eu, ev, ecols,
                      d.removed, aoff + 1);
   // exclusive scan of degrees on the host (nv is the vertex count; small)
@@ -2327,10 +2384,10 @@ int Engine::guided_clear(Dev& d, int64_t n, int32_t maxdim, const int32_t* d_dim
   rc = ensure(b_hash_, (size_t)(hsize * (sizeof(Key128) + sizeof(int32_t))), (uint8_t**)&keys);
   if (rc) return rc;
   int32_t* vals = (int32_t*)(keys + hsize);
-  HIPCHK(hipMemsetAsync(vals, 0xff, sizeof(int32_t) * hsize, stream_));
+  DCHK(dfill(vals, 0xff, sizeof(int32_t) * hsize));
   hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(n)), dim3(256), 0, stream_, n, maxdim, d_dims, d.removed, vs, bits,
                      keys, vals, (uint64_t)(hsize - 1));
-  HIPCHK(hipMemsetAsync(d.ctr + cw::kHdCleared, 0, sizeof(uint32_t), stream_));
+  DCHK(dfill(d.ctr + cw::kHdCleared, 0, sizeof(uint32_t)));
   hipLaunchKernelGGL(k_hd_clear, dim3(grid_for(n)), dim3(256), 0, stream_, n, maxdim, d_dims, d.removed, vs, bits,
                      aoff, adj, keys, vals, (uint64_t)(hsize - 1), p.max_iteration, hd, d.ctr + cw::kHdCleared);
   HIPCHK(hipGetLastError());
@@ -2403,7 +2460,7 @@ int Engine::dual_by_sort(Dev& d, int64_t n, int64_t nnz, const int64_t* colptr, 
     err_ = "radix sort failed";
     return SSQ_EDEVICE;
   }
-  HIPCHK(hipMemsetAsync(len, 0, sizeof(int32_t) * n, stream_));
+  DCHK(dfill(len, 0, sizeof(int32_t) * n));
   const int ge = grid_for((nnz + 3) / 4);
   hipLaunchKernelGGL(k_run_first, dim3(ge), dim3(256), 0, stream_, n, nnz, keys_out, dcolptr);
   hipLaunchKernelGGL(k_run_len, dim3(ge), dim3(256), 0, stream_, n, nnz, keys_out, dcolptr, len);
@@ -2430,21 +2487,21 @@ int Engine::build_dual(Dev& d, int64_t n, int64_t nnz, int32_t maxdim, const int
   if (sorted) {
     hipLaunchKernelGGL(k_dual_dims, dim3(g), dim3(256), 0, stream_, n, dims, maxdim, d.removed, *ddims, drem);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(d.removed, drem, (size_t)n, hipMemcpyDeviceToDevice, stream_));
+    DCHK(dcopy(d.removed, drem, (size_t)n));
     return SSQ_OK;
   }
-  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
+  DCHK(dfill(cnt, 0, sizeof(int32_t) * n));
   hipLaunchKernelGGL(k_dual_count, dim3(g), dim3(256), 0, stream_, n, colptr, rows, d.removed, cnt);
   rc = scan_counts(cnt, n, *dcolptr);
   if (rc) return rc;
-  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, stream_));
+  DCHK(dfill(cnt, 0, sizeof(int32_t) * n));
   hipLaunchKernelGGL(k_dual_fill, dim3(g), dim3(256), 0, stream_, n, colptr, rows, d.removed, *dcolptr, cnt, *drows);
   const int64_t sort_groups = (n + 4 * kWave - 1) / (4 * kWave);
   hipLaunchKernelGGL(k_sort_segments, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(sort_groups, (int64_t)cus_ * 32))),
                      dim3(256), 0, stream_, n, *dcolptr, *drows);
   hipLaunchKernelGGL(k_dual_dims, dim3(g), dim3(256), 0, stream_, n, dims, maxdim, d.removed, *ddims, drem);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(d.removed, drem, (size_t)n, hipMemcpyDeviceToDevice, stream_));
+  DCHK(dcopy(d.removed, drem, (size_t)n));
   return SSQ_OK;
 }
 
@@ -2462,7 +2519,7 @@ int Engine::init_lists(Dev& d, int32_t maxdim, const uint8_t* hd, bool neg_known
   HIPCHK(hipGetLastError());
   // pending lists per dimension and page bounds, read back in one round trip
   const int pend_keys = std::min(kMaxDims, std::max(maxdim, 1) + 1);  // dims 1..maxdim hold pending columns
-  HIPCHK(hipMemsetAsync(tot_ + kTotPending, 0, sizeof(int64_t) * kMaxDims, stream_));
+  DCHK(dfill(tot_ + kTotPending, 0, sizeof(int64_t) * kMaxDims));
   int rc = compact(d.ncols, pend_keys, KeyPendingDim{d.state, d.dims, pend_keys}, EmitIndex{L->U}, nullptr, kTotPending);
   if (rc) return rc;
 
@@ -2519,7 +2576,7 @@ int Engine::page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPla
     int64_t mrow = (int64_t)L.maxrow[dim] - ss, xrow = mrow + ss;
     const int64_t dminrow = L.minrow[dim];
     while ((begin < nl || xrow > dminrow) && !oom) {
-      HIPCHK(hipMemsetAsync(d.ctr, 0, 8 * sizeof(uint32_t), stream_));
+      DCHK(dfill(d.ctr, 0, 8 * sizeof(uint32_t)));
       const int64_t wn = end - begin;
       if (wn > 0) {
         const int32_t fl = (int32_t)std::max<int64_t>(mrow, INT32_MIN);
@@ -2625,18 +2682,19 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       (rc = ensure(b_cap_, (size_t)n, &d.cap)) || (rc = ensure(b_low_, (size_t)n, &d.low)) ||
       (rc = ensure(b_state_, (size_t)n, &d.state)) || (rc = ensure(b_removed_, (size_t)n, &d.removed)) ||
       (rc = ensure(b_flag_, (size_t)(2 * n), &d.flag)) || (rc = ensure(b_piv_, (size_t)n, &d.piv)) ||
-      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)cw::kWords, &d.ctr)) ||
+      (rc = ensure(b_left_, (size_t)n, &d.leftone)) ||
       (rc = ensure(b_pmeta_, (size_t)n, &d.pivmeta)))
     return rc;
+  d.ctr = ctr_;  // the counter block is the engine's own (read back by the host every few rounds)
   int32_t* U;
   if ((rc = ensure(b_U_, (size_t)n, &U))) return rc;
-  HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
+  DCHK(dfill(d.ctr, 0, cw::kWords * sizeof(uint32_t)));
   ctr_dev_ = d.ctr;
   d.stats = (unsigned long long*)(d.ctr + cw::kStats);
   d.arena_used = (unsigned long long*)(d.ctr + cw::kArena);
   d.lvl_cnt = d.ctr + cw::kLvl;
   d.err = d.ctr + cw::kErr;
-  HIPCHK(hipMemsetAsync(d.removed, 0, (size_t)n, stream_));
+  DCHK(dfill(d.removed, 0, (size_t)n));
 
   // max dimension (and range check of the cell dimensions)
   int32_t maxdim = 0;
@@ -2679,7 +2737,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     primal_colptr = d_colptr;
     primal_rows = d_rows;
     if ((rc = ensure(b_prem_, (size_t)n, &primal_removed))) return rc;
-    HIPCHK(hipMemcpyAsync(primal_removed, d.removed, (size_t)n, hipMemcpyDeviceToDevice, stream_));
+    DCHK(dcopy(primal_removed, d.removed, (size_t)n));
     rc = build_dual(d, n, nnz, maxdim, d_colptr, d_rows, d_dims, (p.model & 2) != 0, &dcp, &drw, &ddm);
     if (rc) return rc;
     if (p.model & 2) {  // removed cells drop their entries from the dual
@@ -2701,9 +2759,8 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     if ((rc = ensure(b_pool_, (size_t)(nnz_eff + arena_cap), &d.pool))) return rc;
     d.arena_base = nnz_eff;
     d.arena_cap = arena_cap;
-    HIPCHK(hipMemcpyAsync(d.pool, d_rows, sizeof(int32_t) * std::max<int64_t>(nnz_eff, 0), hipMemcpyDeviceToDevice,
-                          stream_));
-    HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
+    DCHK(dcopy(d.pool, d_rows, sizeof(int32_t) * std::max<int64_t>(nnz_eff, 0)));
+    DCHK(dfill(d.ctr, 0, cw::kWords * sizeof(uint32_t)));
     const int g = grid_for(n);
     hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d, 1);
     if (dual)
@@ -2735,7 +2792,7 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       if (rc) return rc;
     } else {  // dual pair (b', d') -> (n-1-d', n-1-b'), sorted by the new birth
       int32_t* death_of = (int32_t*)d.leftone;
-      HIPCHK(hipMemsetAsync(death_of, 0xff, sizeof(int32_t) * n, stream_));
+      DCHK(dfill(death_of, 0xff, sizeof(int32_t) * n));
       hipLaunchKernelGGL(k_dual_pairs_fill, dim3(grid_for(n)), dim3(256), 0, stream_, n, d.piv, death_of);
       rc = compact(n, 1, KeyDeathOf{death_of}, EmitDeathOf{death_of, d_pairs}, nullptr, kTotPairs);
       if (rc) return rc;
@@ -2798,13 +2855,14 @@ int Engine::prebuilt_begin(int64_t n, int64_t ncols, int64_t nnz, int64_t arena,
       (rc = ensure(b_cap_, (size_t)ncols, &d.cap)) || (rc = ensure(b_low_, (size_t)ncols, &d.low)) ||
       (rc = ensure(b_state_, (size_t)ncols, &d.state)) || (rc = ensure(b_removed_, (size_t)ncols, &d.removed)) ||
       (rc = ensure(b_flag_, (size_t)(2 * n), &d.flag)) || (rc = ensure(b_piv_, (size_t)n, &d.piv)) ||
-      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_ctr_, (size_t)cw::kWords, &d.ctr)) ||
+      (rc = ensure(b_left_, (size_t)n, &d.leftone)) ||
       (rc = ensure(b_pmeta_, (size_t)n, &d.pivmeta)) || (rc = ensure(b_pool_, (size_t)(nnz + arena), &d.pool)))
     return rc;
   d.arena_base = nnz;
   d.arena_cap = arena;
-  HIPCHK(hipMemsetAsync(d.removed, 0, (size_t)ncols, stream_));
-  HIPCHK(hipMemsetAsync(d.flag, 0, (size_t)(2 * n), stream_));
+  d.ctr = ctr_;
+  DCHK(dfill(d.removed, 0, (size_t)ncols));
+  DCHK(dfill(d.flag, 0, (size_t)(2 * n)));
   pre_ = d;
   pre_maxdim_ = maxdim;
   out->pool = d.pool;
@@ -2828,13 +2886,14 @@ int Engine::prebuilt_reduce(const ssq_params& p, ssq_stats* st, bool* oom) {
   int32_t* U;
   int rc;
   if ((rc = ensure(b_U_, (size_t)d.ncols, &U))) return rc;
-  HIPCHK(hipMemsetAsync(d.ctr, 0, cw::kWords * sizeof(uint32_t), stream_));
+  d.ctr = ctr_;
+  DCHK(dfill(d.ctr, 0, cw::kWords * sizeof(uint32_t)));
   ctr_dev_ = d.ctr;
   d.stats = (unsigned long long*)(d.ctr + cw::kStats);
   d.arena_used = (unsigned long long*)(d.ctr + cw::kArena);
   d.lvl_cnt = d.ctr + cw::kLvl;
   d.err = d.ctr + cw::kErr;
-  HIPCHK(hipMemsetAsync(d.piv, 0xff, sizeof(uint32_t) * d.n, stream_));
+  DCHK(dfill(d.piv, 0xff, sizeof(uint32_t) * d.n));
   hipLaunchKernelGGL(k_setup, dim3(grid_for(d.ncols)), dim3(256), 0, stream_, d, 0);
   LoopPlan L;
   L.U = U;

@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
+from specseqplusplus_amd import workloads as W  # noqa: E402
+
 METRIC = "pairs bit-exact; diagrams/s + p50/p99: batched server vs plain per-complex GPU"
 HBM_PEAK_GBS = 8000.0
 BATCH_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=64, level2=1024)
@@ -39,33 +41,6 @@ SERVED_MATRIX_PARAMS = dict(dualize=0, model=0, ss_block_size=1 << 30, growth=1.
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def make_cloud(rng, kind, n):
-    if kind == 0:
-        return rng.random((n, 3))
-    if kind == 1:
-        t = rng.random(n) * 2 * np.pi
-        return np.stack([np.cos(t), np.sin(t), np.zeros(n)], 1) + rng.normal(0, 0.05, (n, 3))
-    u = rng.random(n) * 2 * np.pi
-    v = rng.random(n) * 2 * np.pi
-    return np.stack([(1 + 0.3 * np.cos(v)) * np.cos(u), (1 + 0.3 * np.cos(v)) * np.sin(u), 0.3 * np.sin(v)], 1) + \
-        rng.normal(0, 0.03, (n, 3))
-
-
-def percentile_threshold(X, q):
-    from scipy.spatial.distance import pdist
-    return float(np.percentile(pdist(X.astype(np.float64)), q))
-
-
-def build_stream(count, nmin, nmax, seed, q=30.0):
-    """(points float32, threshold) per cloud."""
-    rng = np.random.default_rng(seed)
-    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).astype(int)
-    pts = [make_cloud(rng, i % 3, int(ns[i])).astype(np.float32) for i in range(count)]
-    with ThreadPoolExecutor(8) as ex:
-        rs = list(ex.map(lambda X: percentile_threshold(X, q), pts))
-    return pts, rs
 
 
 # ---------------------------------------------------------------------------------------
@@ -260,26 +235,12 @@ def plain_leg(pts, rs, args):
 
 
 def served_pool(seed, n_clouds, n_mats):
-    """Requests of the served stream and their reference pairs from the oracle.  Clouds: 64-4096
-    points (log-uniform, the three distributions), threshold the 30th percentile of pairwise
-    distances lowered, for large clouds, to a mean degree of 20; matrices: alpha complexes of
-    1e4-1e6 cells (the alpha generator at C3's density)."""
+    """Requests of the served stream (workloads.served_requests: clouds of 64-4096 points, large
+    ones at a mean degree of at most 20; alpha matrices of 1e4-1e6 cells) and their reference
+    pairs from the oracle."""
     import specseqplusplus_amd as S
     from oracle import oracle as O
-    from specseqplusplus_amd.alpha import alpha_complex
-    rng = np.random.default_rng(seed)
-    pts = []
-    for i in range(n_clouds):
-        n = int(np.exp(rng.uniform(np.log(64), np.log(4096))))
-        X = make_cloud(rng, i % 3, n).astype(np.float32)
-        q = min(30.0, 100.0 * 20.0 / (n - 1))
-        pts.append((X, percentile_threshold(X, q)))
-    mats = []
-    for i in range(n_mats):
-        cols = np.exp(rng.uniform(np.log(1e4), np.log(1e6)))
-        npts = max(64, int(cols / 28.5))
-        P = rng.random((npts, 3)) * 100.0 * (npts / 1e5) ** (1 / 3)
-        mats.append(alpha_complex(P, 90.0))
+    pts, mats = W.served_requests(seed, n_clouds, n_mats)
 
     def ref_pts(c):
         m = S.vr_build(c[0], c[1], 2)
@@ -465,7 +426,7 @@ def main():
 
     import specseqplusplus_amd as S
     t = time.time()
-    pts, rs = build_stream(args.complexes, args.nmin, args.nmax, args.seed)
+    pts, rs = W.batched_stream(args.complexes, args.nmin, args.nmax, args.seed)
     log(f"generated {len(pts)} clouds in {time.time() - t:.1f} s")
     h = S.Handle(0)
     B = batched_leg(h, pts, rs, args)

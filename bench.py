@@ -46,10 +46,11 @@ def log(msg):
 # ---------------------------------------------------------------------------------------
 # roofline
 # ---------------------------------------------------------------------------------------
-# Column-addition kernels: (name, time field, algorithmic-bytes field, additions field).  One
-# addition a <- a xor b costs 4*(|a| + |b| + |a xor b|) + 24 algorithmic bytes (read both
-# columns, write the result, plus pivot lookups and column metadata); the engine sums them per
-# kernel.
+# Column-addition kernels: (name, time field, algorithmic-bytes field).  The engine sums the
+# algorithmic bytes per kernel.  A merge a <- a xor b of sorted lists (k_add, and the tiled
+# merges) costs 4*(|a| + |b| + |a xor b|) + 24 bytes: both columns read, the result written, the
+# pivot lookup and column metadata.  An addition into a bit set kept on chip (k_add_long, the
+# window of k_add_huge) costs 4*|b| + 24: only the pivot's rows are read.
 ADD_KERNELS = [("k_add", "t_add_kernel_ms", "add_bytes"),
                ("k_add_long", "t_add_long_kernel_ms", "dense_bytes"),
                ("k_add_huge", "t_add_huge_kernel_ms", "huge_bytes")]

@@ -85,10 +85,11 @@ typedef struct ssq_stats {
   double t_add_long_kernel_ms; /* summed duration of the long-column kernel launches     */
   int64_t max_chain;      /* most additions one column made in one round                */
   int64_t max_addition_len; /* largest |a|+|b| of one addition                          */
-  int64_t n_huge_additions; /* additions into a device-memory bit set (columns whose rows
-                               spread past the LDS window)                                */
-  double huge_bytes;      /* bytes those additions move: 12|b|+24 each (the pivot's rows,
-                             and the bit-set words they toggle, read and written)          */
+  int64_t n_huge_additions; /* additions of columns whose rows spread past the LDS window
+                               (an LDS window over a device-memory bit set)                */
+  double huge_bytes;      /* bytes those additions move: 4|b|+24 each (the pivot's rows; the
+                             window is in LDS, toggles below it are device atomics), and
+                             4(|a|+|b|+|a xor b|)+24 for the ones merged tile by tile      */
   double t_add_huge_kernel_ms; /* summed duration of the huge-column kernel launches      */
 } ssq_stats;
 

@@ -1044,28 +1044,133 @@ __global__ void __launch_bounds__(kLongThreads) k_add_long(Dev d, int r, int32_t
 }
 
 // Columns whose rows spread past the LDS window (kLongBits rows): one workgroup per column,
-// the running column kept as a bit set in this workgroup's region of device memory (gspan
-// rows below the column's low), toggled by global atomics.  A region is all zero between
-// columns: reading the column back clears the words it read.  A pivot with rows below the
-// region sends the column to sorted form and tiled merges, ping-ponging between the column's
-// own slot and one scratch slot.
+// taken one at a time (their chases differ in length by orders of magnitude).  The running
+// column is a bit set over a region of gspan rows whose top row is the column's low.  The top
+// kLongBits rows of it (the window) live in LDS; the rows below, the tail, live in this
+// workgroup's region of device memory, with a summary in LDS of which 32-word chunks may be
+// nonzero.  An addition toggles the pivot's rows: LDS atomics in the window, global atomics
+// below it that nothing waits for.  The new low is looked for in the window.  Only when the
+// window has emptied does the workgroup wait for its global toggles, find the tail's highest
+// row through the summary, and move the window down onto it (reading those words into LDS and
+// clearing them).  The next pivot is fetched while the current one is toggled, whenever the
+// column's second row is known (the next low is then the larger of the two second rows when
+// they differ).  A region is all zero between columns.  A pivot with rows below the region
+// sends the column to sorted form and tiled merges, ping-ponging between the column's own slot
+// and one scratch slot.
 __device__ __forceinline__ uint32_t gload(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // past the L1, after atomics
 }
+__device__ __forceinline__ void gxor(uint32_t* p, uint32_t m) {  // no result: nothing waits for it
+  (void)__hip_atomic_fetch_xor(p, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-// The column back from its region in sorted form (words [w_lo, w_hi]), each word read is
-// cleared.  Returns false when the store is exhausted.
-__device__ bool huge_write_back(Dev& d, uint32_t* gb, int32_t lo, int32_t w_lo, int32_t w_hi, int64_t* a_off,
-                                int32_t* a_cap, int32_t* a_len, int32_t* red, int32_t* sh_flag, int64_t* sh_off,
-                                int32_t* sh_cap) {
+// Summary of the tail in LDS: bit j of summ set <=> region words [32j, 32j + 32) (rows 1024j ..
+// 1024j + 1023 above the region's base) may hold set bits; a clear bit means those words are
+// zero.  Every global toggle marks its chunk; searches clear the stale bits they meet.
+constexpr int kChunkWords = 32;
+constexpr int kSumWords = (kHugeBits / 32 / kChunkWords + 31) / 32;  // 4096 words (16 KiB)
+__device__ __forceinline__ void mark_chunk(uint32_t* summ, int32_t g) {
+  atomicOr(&summ[g >> 10], 1u << ((g >> 5) & 31));
+}
+__device__ __forceinline__ bool chunk_marked(const uint32_t* summ, int32_t ch) {
+  return (summ[ch >> 5] >> (ch & 31)) & 1u;
+}
+
+// Wavefront 0: the two highest set bits of the window at or below LDS word top_w (rows
+// wrow0 + 32 * word + bit).  t2 is kUnknown when the window's bottom was reached with t1 only
+// (the tail may hold it) or when it lies more than 8 steps of 64 words below t1.
+constexpr int32_t kUnknown = -3;
+__device__ __forceinline__ void window_top2(const uint32_t* bits, int32_t top_w, int32_t wrow0, int32_t* t1o,
+                                            int32_t* t2o) {
+  const int lane = lane_id();
+  int32_t t1 = -1, t2 = -1, past = 0;
+  int32_t base = top_w;
+  for (; base >= 0; base -= kWave) {  // t1, t2 are uniform in the wavefront: so are the exits
+    const int32_t w = base - lane;    // lane k holds word base - k: rows descend with the lane
+    const uint32_t m = w >= 0 ? bits[w] : 0u;
+    uint64_t nz = __ballot(m != 0);
+    while (nz && t2 < 0) {
+      const int fl = __ffsll((unsigned long long)nz) - 1;
+      nz &= nz - 1;
+      uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, fl);
+      const int32_t row0 = wrow0 + ((base - fl) << 5);
+      while (mm && t2 < 0) {
+        const int h = 31 - __clz(mm);
+        mm &= ~(1u << h);
+        if (t1 < 0) t1 = row0 + h;
+        else t2 = row0 + h;
+      }
+    }
+    if (t2 >= 0) break;
+    if (t1 >= 0 && ++past >= 8) break;
+  }
+  *t1o = t1;
+  *t2o = t2 >= 0 ? t2 : (t1 >= 0 ? kUnknown : -1);
+}
+
+// Wavefront 0: the tail's highest set row below region chunk ch_top + 1 (all global toggles
+// made visible beforehand), -1 when the tail is empty.  Chunks found empty are unmarked.
+__device__ __forceinline__ int32_t tail_top(uint32_t* summ, const uint32_t* gb, int32_t ch_top, int32_t nwords,
+                                            int32_t lo) {
+  const int lane = lane_id();
+  int32_t ch = ch_top;
+  while (ch >= 0) {
+    int32_t found = -1;
+    for (int32_t sb = ch >> 5; sb >= 0 && found < 0; sb -= kWave) {
+      const int32_t sw = sb - lane;
+      uint32_t m = sw >= 0 ? summ[sw] : 0u;
+      if (sw == (ch >> 5) && (ch & 31) != 31) m &= (2u << (ch & 31)) - 1u;
+      const uint64_t nz = __ballot(m != 0u);
+      if (nz) {
+        const int fl = __ffsll((unsigned long long)nz) - 1;
+        const uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, fl);
+        found = ((sb - fl) << 5) + (31 - __clz(mm));
+      }
+    }
+    if (found < 0) return -1;
+    const int32_t w0 = found << 5;
+    const uint32_t x = (lane < kChunkWords && w0 + lane < nwords) ? gload(gb + w0 + lane) : 0u;
+    const uint64_t nzw = __ballot(x != 0u);
+    if (nzw) {
+      const int hi = 63 - __clzll((long long)nzw);
+      const uint32_t xx = (uint32_t)__builtin_amdgcn_readlane((int)x, hi);
+      return lo + ((w0 + hi) << 5) + (31 - __clz(xx));
+    }
+    if (lane == 0) summ[found >> 5] &= ~(1u << (found & 31));
+    ch = found - 1;
+  }
+  return -1;
+}
+
+// The column back in sorted form: the tail's marked chunks of region words [w_lo, ww0) (each
+// word read is cleared), then the window's words up to LDS word top_w.  Returns false when the
+// store is exhausted (the region is cleared all the same).
+__device__ bool huge_write_back(Dev& d, uint32_t* gb, const uint32_t* summ, const uint32_t* bits, int32_t lo,
+                                int32_t ww0, int32_t w_lo, int32_t top_w, int64_t* a_off, int32_t* a_cap,
+                                int32_t* a_len, int32_t* red, int32_t* sh_flag, int64_t* sh_off, int32_t* sh_cap) {
   const int tid = threadIdx.x;
-  const int32_t W = w_hi >= w_lo ? w_hi - w_lo + 1 : 0;
-  const int32_t per = (W + kLongThreads - 1) / kLongThreads;
-  const int32_t v0 = w_lo + min(W, tid * per), v1 = w_lo + min(W, tid * per + per);
-  int32_t pc = 0;
-  for (int32_t w = v0; w < v1; w++) pc += __popc(gload(gb + w));
-  int32_t total;
-  const int32_t pos0 = block_excl_scan(pc, red, &total);
+  const int32_t nwords = d.gspan >> 5;
+  __threadfence();  // the last addition's global toggles, before the tail is read
+  __syncthreads();
+  // tail chunks [c_lo, c_hi] (ww0 is chunk-aligned: the tail ends at a chunk boundary)
+  const int32_t c_lo = max(w_lo, 0) >> 5, c_hi = (ww0 >> 5) - 1;
+  const int32_t C = c_hi >= c_lo ? c_hi - c_lo + 1 : 0;
+  const int32_t per = (C + kLongThreads - 1) / kLongThreads;
+  const int32_t v0 = c_lo + min(C, tid * per), v1 = c_lo + min(C, tid * per + per);
+  const int32_t WW = top_w >= 0 ? top_w + 1 : 0;  // window words in use
+  const int32_t wper = (WW + kLongThreads - 1) / kLongThreads;
+  const int32_t u0 = min(WW, tid * wper), u1 = min(WW, tid * wper + wper);
+  int32_t pt = 0, pw = 0;
+  for (int32_t ch = v0; ch < v1; ch++) {
+    if (!chunk_marked(summ, ch)) continue;
+    const int32_t ke = min(kChunkWords, nwords - (ch << 5));
+    for (int32_t k = 0; k < ke; k++) pt += __popc(gload(gb + (ch << 5) + k));
+  }
+  for (int32_t w = u0; w < u1; w++) pw += __popc(bits[w]);
+  int32_t total_t, total_w;
+  const int32_t post = block_excl_scan(pt, red, &total_t);
+  const int32_t posw = block_excl_scan(pw, red, &total_w);
+  const int32_t total = total_t + total_w;
   if (tid == 0) {
     int64_t o = *a_off;
     int32_t cp = *a_cap;
@@ -1077,18 +1182,34 @@ __device__ bool huge_write_back(Dev& d, uint32_t* gb, int32_t lo, int32_t w_lo, 
   __syncthreads();
   const bool ok = *sh_flag != 0;
   int32_t* O = d.pool + *sh_off;
-  int32_t pos = pos0;
-  for (int32_t w = v0; w < v1; w++) {
-    uint32_t x = gload(gb + w);
-    if (!x) continue;
-    gb[w] = 0u;
-    if (ok)
+  int32_t pos = post;
+  for (int32_t ch = v0; ch < v1; ch++) {
+    if (!chunk_marked(summ, ch)) continue;
+    const int32_t ke = min(kChunkWords, nwords - (ch << 5));
+    for (int32_t k = 0; k < ke; k++) {
+      const int32_t g = (ch << 5) + k;
+      uint32_t x = gload(gb + g);
+      if (!x) continue;
+      gb[g] = 0u;
+      if (ok)
+        while (x) {
+          const int b = __ffs(x) - 1;
+          O[pos++] = lo + (g << 5) + b;
+          x &= x - 1;
+        }
+    }
+  }
+  pos = total_t + posw;
+  const int32_t wrow0 = lo + (ww0 << 5);
+  if (ok)
+    for (int32_t w = u0; w < u1; w++) {
+      uint32_t x = bits[w];
       while (x) {
         const int b = __ffs(x) - 1;
-        O[pos++] = lo + (w << 5) + b;
+        O[pos++] = wrow0 + (w << 5) + b;
         x &= x - 1;
       }
-  }
+    }
   __threadfence();
   __syncthreads();
   if (ok) {
@@ -1099,10 +1220,17 @@ __device__ bool huge_write_back(Dev& d, uint32_t* gb, int32_t lo, int32_t w_lo, 
   return ok;
 }
 
+// the window's first region word for a low in region word lw: lw's chunk is the window's top
+__device__ __forceinline__ int32_t window_base(int32_t lw) {
+  return max(0, ((lw >> 5) + 1) * kChunkWords - kLongWords);
+}
+
 __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t minrow) {
+  __shared__ uint32_t bits[kLongWords];
+  __shared__ uint32_t summ[kSumWords];
   __shared__ int32_t stage[3][kTileBuf];
   __shared__ int32_t red[2 * (kLongThreads / kWave)];
-  __shared__ int32_t sh_count, sh_flag, sh_minw, sh_cap;
+  __shared__ int32_t sh_flag, sh_minw, sh_cap;
   __shared__ int64_t sh_off;
   __shared__ uint32_t sh_item;
   const int nx = (r + 1) & 1;
@@ -1110,9 +1238,11 @@ __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t
   const uint32_t cnt = d.ctr[cw::kHuge];
   const int32_t* list = d.work[5];
   uint32_t* gb = d.gbits + (int64_t)blockIdx.x * (d.gspan >> 5);
+  const int32_t nwords = d.gspan >> 5;
+  const int32_t nsum = ((nwords + kChunkWords - 1) / kChunkWords + 31) >> 5;
   unsigned long long n_add = 0, n_tiled = 0, n_bytes = 0;
   uint32_t max_chain = 0, max_len = 0;
-  for (;;) {  // columns taken one at a time: their chases differ in length by orders of magnitude
+  for (;;) {
     if (tid == 0) sh_item = atomicAdd(&d.ctr[cw::kHugeNext], 1u);
     __syncthreads();
     const uint32_t it = sh_item;
@@ -1128,78 +1258,156 @@ __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t
     uint32_t chain = 0;
     int64_t t_off = 0;  // scratch slot of the tiled merges (thread 0's copy is used)
     int32_t t_cap = 0;
+    int32_t ww0 = window_base((l - lo) >> 5);  // region word of LDS word 0
+    int32_t a2 = kUnknown;                     // the running column's second row
+    int32_t spec_row = -2;
+    uint32_t spec_p = kEmpty;
+    PivMeta spec_pm{};
     if (dense) {
+      for (int32_t k = tid; k < kLongWords; k += kLongThreads) bits[k] = 0u;
+      for (int32_t k = tid; k < nsum; k += kLongThreads) summ[k] = 0u;
+      __syncthreads();
       const int32_t* A = d.pool + a_off;
       for (int32_t k = tid; k < a_len; k += kLongThreads) {
-        const int32_t x = A[k] - lo;
-        atomicOr(gb + (x >> 5), 1u << (x & 31));
+        const int32_t x = A[k] - lo, g = x >> 5;
+        if (g >= ww0) {
+          atomicOr(&bits[g - ww0], 1u << (x & 31));
+        } else {
+          atomicOr(gb + g, 1u << (x & 31));
+          mark_chunk(summ, g);
+        }
       }
-      if (tid == 0) {
-        sh_count = a_len;
-        sh_minw = (A[0] - lo) >> 5;
-      }
+      if (tid == 0) sh_minw = (A[0] - lo) >> 5;
       __threadfence();
       __syncthreads();
+      a2 = a_len >= 2 ? A[a_len - 2] : -1;
     }
     for (;;) {
       l = __builtin_amdgcn_readfirstlane(l);
-      const uint32_t p = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.piv[l]);
+      uint32_t p;
+      PivMeta pm;
+      if (l == spec_row) {
+        p = spec_p;
+        pm = spec_pm;
+      } else {
+        p = d.piv[l];
+        pm = d.pivmeta[l];  // fetched with the pivot (meaningful only if it exists)
+      }
+      p = (uint32_t)__builtin_amdgcn_readfirstlane((int)p);
       if (p == kEmpty || p >= (uint32_t)c) break;
-      const PivMeta pm = d.pivmeta[l];
       const int32_t b_len = __builtin_amdgcn_readfirstlane(pm.len);
       const int32_t b_first = __builtin_amdgcn_readfirstlane(pm.first);
+      const int32_t b2 = __builtin_amdgcn_readfirstlane(pm.second);
       const int64_t b_off = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(pm.off & 0xffffffff)) |
                             ((int64_t)__builtin_amdgcn_readfirstlane((int)(pm.off >> 32)) << 32);
       const int32_t* B = d.pool + b_off;
       if (dense && b_first < lo) {  // leave the region: sorted form, then tiled merges
-        if (!huge_write_back(d, gb, lo, sh_minw, (l - lo) >> 5, &a_off, &a_cap, &a_len, red, &sh_flag, &sh_off,
-                             &sh_cap)) {
+        if (!huge_write_back(d, gb, summ, bits, lo, ww0, sh_minw, ((l - lo) >> 5) - ww0, &a_off, &a_cap, &a_len,
+                             red, &sh_flag, &sh_off, &sh_cap)) {
           failed = true;
           break;
         }
         dense = false;
       }
-      int32_t r_len;
       if (dense) {
-        int32_t delta = 0;
-        for (int32_t k = tid; k < b_len; k += kLongThreads) {
-          const int32_t x = B[k] - lo;
-          const uint32_t m = 1u << (x & 31);
-          const uint32_t old = atomicXor(gb + (x >> 5), m);
-          delta += (old & m) ? -1 : 1;
-        }
-        for (int k = kWave / 2; k > 0; k >>= 1) delta += __shfl_xor(delta, k, kWave);
-        if (lane_id() == 0) atomicAdd(&sh_count, delta);
-        if (tid == 0) atomicMin(&sh_minw, (b_first - lo) >> 5);
-        __threadfence();
-        __syncthreads();
-        r_len = sh_count;
-        // new low: the highest set bit at or below the old low, wavefront 0 scanning down
-        // 64 words per step
-        if (tid < kWave) {
-          int32_t top = -1;
-          for (int32_t base = r_len > 0 ? (l - lo) >> 5 : -1; base >= 0; base -= kWave) {
-            const int32_t w = base - (int32_t)tid;
-            const uint32_t m = w >= 0 ? gload(gb + w) : 0u;
-            const uint64_t nz = __ballot(m != 0);
-            if (nz) {
-              const int fl = __ffsll((unsigned long long)nz) - 1;
-              const uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, fl);
-              top = lo + ((base - fl) << 5) + (31 - __clz(mm));
-              break;
-            }
+        spec_row = -2;
+        if (a2 != kUnknown && a2 != b2) {
+          const int32_t cand = max(a2, b2);
+          if (cand >= 0 && cand >= minrow) {
+            spec_row = cand;
+            spec_p = d.piv[cand];
+            spec_pm = d.pivmeta[cand];
           }
-          if (tid == 0) red[0] = top;
+        }
+        // toggle the pivot's rows
+        constexpr int kU = 16;  // independent loads in flight per thread
+        for (int32_t k0 = tid; k0 < b_len; k0 += kLongThreads * kU) {
+          int32_t x[kU];
+#pragma unroll
+          for (int u = 0; u < kU; u++) {
+            const int32_t k = k0 + u * kLongThreads;
+            x[u] = k < b_len ? B[k] - lo : -1;
+          }
+#pragma unroll
+          for (int u = 0; u < kU; u++)
+            if (x[u] >= 0) {
+              const int32_t g = x[u] >> 5;
+              const uint32_t m = 1u << (x[u] & 31);
+              if (g >= ww0) {
+                atomicXor(&bits[g - ww0], m);
+              } else {
+                gxor(gb + g, m);
+                mark_chunk(summ, g);
+              }
+            }
+        }
+        if (tid == 0) atomicMin(&sh_minw, (b_first - lo) >> 5);
+        __syncthreads();
+        // new low and the row below it, in the window
+        if (tid < kWave) {
+          int32_t t1, t2;
+          window_top2(bits, ((l - lo) >> 5) - ww0, lo + (ww0 << 5), &t1, &t2);
+          if (tid == 0) {
+            red[0] = t1;
+            red[1] = t2;
+          }
         }
         __syncthreads();
-        const int32_t newl = red[0];
+        int32_t newl = red[0];
+        a2 = red[1];
+        if (newl < 0 && ww0 > 0) {
+          // the window is empty: wait for the global toggles, take the tail's highest row and
+          // move the window onto it
+          __threadfence();
+          __syncthreads();
+          if (tid < kWave) {
+            const int32_t t = tail_top(summ, gb, (ww0 >> 5) - 1, nwords, lo);
+            if (tid == 0) red[0] = t;
+          }
+          __syncthreads();
+          newl = red[0];
+          if (newl >= 0) {
+            const int32_t lw = (newl - lo) >> 5;
+            const int32_t nb = window_base(lw);  // chunk-aligned, below the old window
+            // pull the marked chunks [nb >> 5, lw >> 5] into the (all zero) window
+            const int32_t c0 = nb >> 5, C = (lw >> 5) - c0 + 1;
+            for (int32_t ci = tid; ci < C; ci += kLongThreads) {
+              const int32_t ch = c0 + ci;
+              if (!chunk_marked(summ, ch)) continue;
+              const int32_t ke = min(kChunkWords, nwords - (ch << 5));
+              for (int32_t k = 0; k < ke; k++) {
+                const int32_t g = (ch << 5) + k;
+                const uint32_t x = gload(gb + g);
+                bits[g - nb] = x;
+                if (x) gb[g] = 0u;
+              }
+            }
+            __syncthreads();
+            for (int32_t ci = tid; ci < C; ci += kLongThreads) {  // every pulled chunk is now zero
+              const int32_t ch = c0 + ci;
+              if (chunk_marked(summ, ch)) atomicAnd(&summ[ch >> 5], ~(1u << (ch & 31)));
+            }
+            ww0 = nb;
+            __syncthreads();
+            if (tid < kWave) {
+              int32_t t1, t2;
+              window_top2(bits, lw - ww0, lo + (ww0 << 5), &t1, &t2);
+              if (tid == 0) {
+                red[0] = t1;
+                red[1] = t2;
+              }
+            }
+            __syncthreads();
+            newl = red[0];
+            a2 = red[1];
+          }
+        }
         n_add += 1;
-        n_bytes += 12ull * (unsigned long long)b_len + 24ull;  // the pivot read, its words toggled (read, write)
-        max_len = max(max_len, (uint32_t)(a_len + b_len));
+        n_bytes += 4ull * (unsigned long long)b_len + 24ull;  // the pivot read; the bit set stays on chip
+        max_len = max(max_len, (uint32_t)b_len);
         chain += 1;
-        a_len = r_len;
-        l = r_len ? newl : -1;
-        __syncthreads();  // red[0] read by all before the next scan writes it
+        l = newl;
+        __syncthreads();  // red[] read by all before the next search writes it
       } else {
         // tiled merge by wave 0 into the other of two slots: the column's own and one scratch
         // slot, swapped after each addition; the scratch slot is replaced (twice the need)
@@ -1223,15 +1431,16 @@ __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t
           failed = true;
           break;
         }
+        int32_t r_len = 0;
         if (tid < kWave)
           r_len = wave_tiled_xor(d.pool + a_off, a_len, B, b_len, d.pool + sh_off, stage[0], stage[1], stage[2]);
         if (tid == 0) {
-          sh_count = r_len;
+          red[0] = r_len;
           t_off = a_off;  // the column's previous storage becomes the scratch slot
           t_cap = a_cap;
         }
         __syncthreads();
-        r_len = sh_count;
+        r_len = red[0];
         n_tiled += 1;
         n_add += 1;
         n_bytes += 4ull * (unsigned long long)(a_len + b_len + r_len) + 24ull;
@@ -1246,12 +1455,14 @@ __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t
       if (l < 0 || l < minrow) break;
     }
     max_chain = max(max_chain, chain);
-    if (dense && !failed && a_len > 0) {
-      if (!huge_write_back(d, gb, lo, sh_minw, (l - lo) >> 5, &a_off, &a_cap, &a_len, red, &sh_flag, &sh_off,
-                           &sh_cap))
-        failed = true;
-    } else if (dense && failed && a_len > 0) {  // keep the region zero for the next column
-      huge_write_back(d, gb, lo, sh_minw, (l - lo) >> 5, &a_off, &a_cap, &a_len, red, &sh_flag, &sh_off, &sh_cap);
+    if (dense && !failed) {  // sorted form back; the region ends all zero
+      if (l >= 0) {
+        if (!huge_write_back(d, gb, summ, bits, lo, ww0, sh_minw, ((l - lo) >> 5) - ww0, &a_off, &a_cap, &a_len,
+                             red, &sh_flag, &sh_off, &sh_cap))
+          failed = true;
+      } else {
+        a_len = 0;  // every row cancelled: the window and the tail are zero
+      }
     }
     if (tid == 0) {
       if (failed) {

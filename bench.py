@@ -90,7 +90,7 @@ def roofline_record(st):
 # legs
 # ---------------------------------------------------------------------------------------
 def batched_leg(h, pts, rs, args):
-    """The headline: every cloud's pairs, points resident in HBM, three streams."""
+    """The headline: every cloud's pairs, points resident in HBM, four streams."""
     import specseqplusplus_amd as S
     pbase = np.concatenate([[0], np.cumsum([len(X) for X in pts])]).astype(np.int64)
     d_pts = h.device_array(np.concatenate(pts).astype(np.float32))
@@ -471,7 +471,7 @@ def main():
         "config": {"workload": f"batched: {args.complexes} Rips complexes, n log-uniform [{args.nmin},{args.nmax}], "
                                f"cube/circle/torus, r = 30th percentile, dim<=2; points in HBM -> pairs in HBM",
                    "points": int(B["pbase"][-1]), "edges": int(B["edges"].sum()), "triangles": int(B["tris"].sum()),
-                   "cells": N, "nnz": NNZ, "pairs": npairs, "streams": 3, "groups_per_step": int(last["n_pages"]),
+                   "cells": N, "nnz": NNZ, "pairs": npairs, "streams": 4, "groups_per_step": int(last["n_pages"]),
                    "params": BATCH_PARAMS},
         "plain_gpu_diagrams_per_s": round(plain["plain_gpu"], 3) if plain else None,
         "batched_over_plain": round(B["diagrams_per_s"] / plain["plain_gpu"], 2) if plain else None,

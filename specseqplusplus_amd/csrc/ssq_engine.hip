@@ -3138,7 +3138,7 @@ struct Handle {
   // further engines and builders of the batched Rips path: its groups run on several streams
   std::vector<std::unique_ptr<Engine>> xeng;
   std::vector<RipsCo*> xrc;
-  int rips_streams = 0;  // 0: SSQ_RIPS_STREAMS or 3
+  int rips_streams = 0;  // 0: SSQ_RIPS_STREAMS or 4
   DevBuf pts;
   explicit Handle(int dev) : eng(dev) {}
   ~Handle() {
@@ -3479,10 +3479,10 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
 }
 
 // Streams of the batched Rips path: the handle's setting (ssq::handle_set_rips_streams), else
-// SSQ_RIPS_STREAMS (1-4), else 3.  HIP places a stream on one of GPU_MAX_HW_QUEUES hardware
-// queues (4 by default), so at most 4.
+// SSQ_RIPS_STREAMS (1-4), else 4.  HIP places a stream on one of GPU_MAX_HW_QUEUES hardware
+// queues (4 by default), so at most 4: one group per hardware queue.
 static int rips_streams(const Handle* H) {
-  int n = 3;
+  int n = 4;
   if (const char* e = std::getenv("SSQ_RIPS_STREAMS")) n = std::atoi(e);
   if (H->rips_streams > 0) n = H->rips_streams;
   return std::max(1, std::min(4, n));

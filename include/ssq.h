@@ -114,6 +114,17 @@ const char* ssq_last_error(const ssq_handle* h);
 int ssq_reduce(ssq_handle* h, const ssq_matrix* m, const ssq_params* p, ssq_pairs* out,
                ssq_stats* stats);
 
+/* The same reduction in the reference's two phases, for callers that time or schedule them apart
+ * (main.cu:337-348 calls GPU_Init, then GPU_Compute).  ssq_init uploads the matrix and runs
+ * specseq.h:513-590 GPU_Init's work: edge collapse (model bit 1), high-dim clearing (model bit 0)
+ * or the anti-transpose (dualize), the init passes (initleft/initlookup/clearing/compress) and the
+ * pending lists; stats->t_init_ms is its time.  ssq_compute runs specseq.h:592-1060 GPU_Compute
+ * (the page loop) and returns the pairs as ssq_reduce does; stats->t_reduce_ms is its time.  A
+ * handle holds one initialized matrix; any other reduction on the handle discards it, and
+ * ssq_compute without one returns SSQ_EINVAL. */
+int ssq_init(ssq_handle* h, const ssq_matrix* m, const ssq_params* p, ssq_stats* stats);
+int ssq_compute(ssq_handle* h, ssq_pairs* out, ssq_stats* stats);
+
 /* Same, with the matrix already resident in device memory (device pointers d_col_ptr
  * [n+1], d_rows[nnz], d_dims[n]).  Pairs are written to device memory d_pairs_out (2*n
  * int64 capacity) and their count to *np_out.  Used by the timed region of bench.py. */

@@ -17,9 +17,10 @@
  *                    Pinned by: the single_triangle KAT; agreement with or_dense_standard
  *                    (standard algorithm, no clearing) and with or_rank_pairs (pivots from
  *                    ranks of lower-left submatrices, an algorithm-free characterisation) on
- *                    random tiny complexes; Betti numbers of triangulated spheres / tori /
- *                    projective plane (closed forms); pair counts of the shipped _maxclear
- *                    fixtures equal those of their originals (data/Alpha, byte-identical).
+ *                    random tiny complexes and with or_dense_standard on the reference's
+ *                    shipped 20,875-cell data/ matrices; Betti numbers of triangulated spheres /
+ *                    tori / projective plane (closed forms).  (The shipped Alpha *_maxclear files
+ *                    are byte-identical to their originals and add no pin of their own.)
  *  or_twist_reduce_dual  specseq.h:303-363,1188 (dualize: twist reduction of the
  *                    anti-transpose, pairs mapped back as (b', d') -> (n-1-d', n-1-b')).
  *                    Pinned by agreement with or_twist_reduce (persistence duality: homology
@@ -34,7 +35,12 @@
  *                    Pinned against oracle/_ref (the reference header compiled from its own
  *                    source) and the committed kept-edge fixtures under tests/golden/.
  *  or_apply_collapse specseq.h:417-423 + kernel.h:190-198 (non-kept edges and all their
- *                    cofaces are emptied).  Pinned by the sub-complex invariants in tests.
+ *                    cofaces are emptied).  Pinned by the sub-complex invariants and by
+ *                    homotopy invariance: a strong (edge) collapse keeps the flag complex's
+ *                    homotopy type, so the kept sub-complex has the full complex's beta_0 and
+ *                    beta_1 (the 2-skeleton's beta_2 need not agree); a planted over-removal
+ *                    fails it.  (The kept edges' changed critical times that GUDHI reports are
+ *                    dropped by the reference, specseq.h:421-423, so bars are not preserved.)
  */
 #ifndef SSQ_ORACLE_H
 #define SSQ_ORACLE_H

@@ -64,6 +64,9 @@ SIGNATURES = {
     "ssq_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "ssq_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_matrix), ctypes.POINTER(ssq_params),
                                   ctypes.POINTER(ssq_pairs), ctypes.POINTER(ssq_stats)]),
+    "ssq_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_matrix), ctypes.POINTER(ssq_params),
+                                ctypes.POINTER(ssq_stats)]),
+    "ssq_compute": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_pairs), ctypes.POINTER(ssq_stats)]),
     "ssq_reduce_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ssq_params),
                                          ctypes.c_void_p, i64p, ctypes.POINTER(ssq_stats)]),

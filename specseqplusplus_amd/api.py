@@ -125,6 +125,22 @@ class Handle:
         pairs, ess = _pairs_from_c(pr)
         return (pairs, ess, st.as_dict()) if stats else (pairs, ess)
 
+    def init(self, m, p=None):
+        """Phase one (ssq_init): upload and prepare m; returns the stats (t_init_ms)."""
+        p = p or params()
+        st = L.ssq_stats()
+        ms = m.c_struct()
+        L.check(L.lib().ssq_init(self._h, ctypes.byref(ms), ctypes.byref(p), ctypes.byref(st)), self.error())
+        return st.as_dict()
+
+    def compute(self, stats=False):
+        """Phase two (ssq_compute): the page loop on the matrix init() left; (pairs, essential)."""
+        pr = L.ssq_pairs()
+        st = L.ssq_stats()
+        L.check(L.lib().ssq_compute(self._h, ctypes.byref(pr), ctypes.byref(st)), self.error())
+        pairs, ess = _pairs_from_c(pr)
+        return (pairs, ess, st.as_dict()) if stats else (pairs, ess)
+
     def reduce_device(self, n, nnz, d_colptr, d_rows, d_dims, d_pairs, p=None):
         """Device-resident reduction: arguments are device pointers (ints)."""
         p = p or params()

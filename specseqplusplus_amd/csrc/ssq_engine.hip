@@ -2194,6 +2194,12 @@ class Engine : public PrebuiltTarget {
   int reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
                     const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
                     ssq_stats* st, int64_t* d_essential = nullptr, int64_t* ne_out = nullptr);
+  // The same in the reference's two phases (GPU_Init, then GPU_Compute and the pairs): the engine
+  // holds one initialized matrix between them; any other reduction on the engine discards it.
+  int init_phase(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows, const int32_t* d_dims,
+                 const ssq_params& p, ssq_stats* st);
+  int compute_phase(int64_t* d_pairs, int64_t* np_out, ssq_stats* st, int64_t* d_essential = nullptr,
+                    int64_t* ne_out = nullptr);
   // Prebuilt matrices: a builder writes the rows and the init tables straight into the engine's
   // buffers, then runs the init passes and the page loop on them.  Cells [0, n) are rows;
   // cells [0, ncols) also have columns (colptr[ncols + 1] into the pool, dims[ncols]; both stay
@@ -2327,6 +2333,27 @@ class Engine : public PrebuiltTarget {
   };
   int init_lists(Dev& d, int32_t maxdim, const uint8_t* hd, bool neg_known, const ssq_params& p, LoopPlan* L,
                  ssq_stats* S);
+  // A reduction between its two phases: the matrix as reduced (the anti-transpose when dualize
+  // is set), the column state, the pending lists and the counters so far.
+  struct Phase {
+    bool ready = false;
+    Dev d{};
+    LoopPlan L;
+    ssq_params p{};
+    int32_t maxdim = 0;
+    bool dual = false;
+    int64_t n = 0, nnz_eff = 0, arena_cap = 0;
+    const int32_t* rows = nullptr;  // the rows the column store starts from
+    const int64_t* primal_colptr = nullptr;
+    const int32_t* primal_rows = nullptr;
+    uint8_t* primal_removed = nullptr;
+    uint8_t* hd = nullptr;
+    int32_t* U = nullptr;
+    ssq_stats S{};
+    std::chrono::steady_clock::time_point T0, T1;
+  };
+  Phase ph_;
+  int prepare_store();
   int page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPlan& L, ssq_stats* S, bool* oom_out);
   // The huge-column bit set regions, allocated and zeroed once (k_add_huge leaves them zero).
   int ensure_gbits(Dev* d) {
@@ -2864,9 +2891,22 @@ int Engine::page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPla
 int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
                           const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
                           ssq_stats* st, int64_t* d_essential, int64_t* ne_out) {
-  // (d_colptr, d_rows, d_dims are re-pointed at the dual matrix when p.dualize is set)
+  if (!np_out || (n > 0 && !d_pairs)) {
+    err_ = "invalid arguments";
+    return SSQ_EINVAL;
+  }
+  int rc = init_phase(n, nnz, d_colptr, d_rows, d_dims, p, st);
+  if (rc) return rc;
+  return compute_phase(d_pairs, np_out, st, d_essential, ne_out);
+}
+
+// Phase one (specseq.h:513-590 GPU_Init): validation, buffers, edge collapse, the anti-transpose
+// or guided clearing, then the column store, setup, init passes and pending lists.
+int Engine::init_phase(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows, const int32_t* d_dims,
+                       const ssq_params& p, ssq_stats* st) {
   err_.clear();
-  if (n < 0 || nnz < 0 || n >= 0x7fffffffLL || !np_out || (n > 0 && (!d_colptr || !d_dims || !d_pairs))) {
+  ph_.ready = false;
+  if (n < 0 || nnz < 0 || n >= 0x7fffffffLL || (n > 0 && (!d_colptr || !d_dims))) {
     err_ = "invalid arguments";
     return SSQ_EINVAL;
   }
@@ -2876,12 +2916,18 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
     return SSQ_EINVAL;
   }
   HIPCHK(hipSetDevice(device_));
-  auto T0 = std::chrono::steady_clock::now();
-  *np_out = 0;
-  if (ne_out) *ne_out = 0;
-  if (n == 0) return SSQ_OK;
-
-  Dev d{};
+  std::memset(&ph_.S, 0, sizeof(ph_.S));
+  ph_.T0 = std::chrono::steady_clock::now();
+  ph_.n = n;
+  ph_.p = p;
+  if (n == 0) {
+    ph_.ready = true;
+    if (st) *st = ph_.S;
+    return SSQ_OK;
+  }
+  // (d_colptr, d_rows, d_dims are re-pointed at the dual matrix when p.dualize is set)
+  Dev& d = ph_.d;
+  d = Dev{};
   d.n = n;
   d.ncols = n;
   d.short_max = short_max_;
@@ -2893,12 +2939,10 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
       (rc = ensure(b_cap_, (size_t)n, &d.cap)) || (rc = ensure(b_low_, (size_t)n, &d.low)) ||
       (rc = ensure(b_state_, (size_t)n, &d.state)) || (rc = ensure(b_removed_, (size_t)n, &d.removed)) ||
       (rc = ensure(b_flag_, (size_t)(2 * n), &d.flag)) || (rc = ensure(b_piv_, (size_t)n, &d.piv)) ||
-      (rc = ensure(b_left_, (size_t)n, &d.leftone)) ||
-      (rc = ensure(b_pmeta_, (size_t)n, &d.pivmeta)))
+      (rc = ensure(b_left_, (size_t)n, &d.leftone)) || (rc = ensure(b_pmeta_, (size_t)n, &d.pivmeta)) ||
+      (rc = ensure(b_U_, (size_t)n, &ph_.U)))
     return rc;
   d.ctr = ctr_;  // the counter block is the engine's own (read back by the host every few rounds)
-  int32_t* U;
-  if ((rc = ensure(b_U_, (size_t)n, &U))) return rc;
   DCHK(dfill(d.ctr, 0, cw::kWords * sizeof(uint32_t)));
   ctr_dev_ = d.ctr;
   d.stats = (unsigned long long*)(d.ctr + cw::kStats);
@@ -2908,125 +2952,145 @@ int Engine::reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const
   DCHK(dfill(d.removed, 0, (size_t)n));
 
   // max dimension (and range check of the cell dimensions)
-  int32_t maxdim = 0;
-  {
-    hipLaunchKernelGGL(k_maxdim, dim3(grid_for(n)), dim3(256), 0, stream_, n, d_dims,
-                       (int32_t*)(d.ctr + cw::kMaxDim), d.ctr + cw::kMaxDim + 1);
-    HIPCHK(hipGetLastError());
-    rc = sync_read(d.ctr + cw::kMaxDim, 2);
-    if (rc) return rc;
-    if (host_ctr_[1]) {
-      err_ = "cell dimension outside [0, 7]";
-      return SSQ_EFORMAT;
-    }
-    maxdim = (int32_t)host_ctr_[0];
+  hipLaunchKernelGGL(k_maxdim, dim3(grid_for(n)), dim3(256), 0, stream_, n, d_dims, (int32_t*)(d.ctr + cw::kMaxDim),
+                     d.ctr + cw::kMaxDim + 1);
+  HIPCHK(hipGetLastError());
+  if ((rc = sync_read(d.ctr + cw::kMaxDim, 2))) return rc;
+  if (host_ctr_[1]) {
+    err_ = "cell dimension outside [0, 7]";
+    return SSQ_EFORMAT;
   }
+  ph_.maxdim = (int32_t)host_ctr_[0];
 
   // column store: input rows first, then the arena
-  const int64_t arena0 = std::max<int64_t>({arena_hint_, 4 * nnz + 4096, (int64_t)1 << 22});
-  int64_t arena_cap = arena0;
-
-  ssq_stats local{};
-  ssq_stats* S = st ? st : &local;
-  std::memset(S, 0, sizeof(*S));
-
-  if ((p.model & 2) && maxdim >= 1) {
-    rc = collapse(d, n, d_dims, d_colptr, d_rows, S);
-    if (rc) return rc;
-  }
-  uint8_t* hd = nullptr;
-  const bool dual = p.dualize == 1;
-  const int64_t* primal_colptr = nullptr;
-  const int32_t* primal_rows = nullptr;
-  uint8_t* primal_removed = nullptr;
-  int64_t nnz_eff = nnz;
-  if (dual) {
+  ph_.arena_cap = std::max<int64_t>({arena_hint_, 4 * nnz + 4096, (int64_t)1 << 22});
+  ssq_stats* S = &ph_.S;
+  if ((p.model & 2) && ph_.maxdim >= 1 && (rc = collapse(d, n, d_dims, d_colptr, d_rows, S))) return rc;
+  ph_.hd = nullptr;
+  ph_.dual = p.dualize == 1;
+  ph_.primal_colptr = nullptr;
+  ph_.primal_rows = nullptr;
+  ph_.primal_removed = nullptr;
+  ph_.nnz_eff = nnz;
+  ph_.rows = d_rows;
+  if (ph_.dual) {
     // cohomology on the (collapsed) complex: reduce the anti-transpose; guided clearing is a
     // statement about the top dimension of the primal complex and is not used here
     int64_t* dcp;
     int32_t *drw, *ddm;
-    primal_colptr = d_colptr;
-    primal_rows = d_rows;
-    if ((rc = ensure(b_prem_, (size_t)n, &primal_removed))) return rc;
-    DCHK(dcopy(primal_removed, d.removed, (size_t)n));
-    rc = build_dual(d, n, nnz, maxdim, d_colptr, d_rows, d_dims, (p.model & 2) != 0, &dcp, &drw, &ddm);
+    ph_.primal_colptr = d_colptr;
+    ph_.primal_rows = d_rows;
+    if ((rc = ensure(b_prem_, (size_t)n, &ph_.primal_removed))) return rc;
+    DCHK(dcopy(ph_.primal_removed, d.removed, (size_t)n));
+    rc = build_dual(d, n, nnz, ph_.maxdim, d_colptr, d_rows, d_dims, (p.model & 2) != 0, &dcp, &drw, &ddm);
     if (rc) return rc;
     if (p.model & 2) {  // removed cells drop their entries from the dual
       HIPCHK(hipMemcpyAsync(host_tot_ + kTotScratch, dcp + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
       HIPCHK(hipStreamSynchronize(stream_));
-      nnz_eff = host_tot_[kTotScratch];
+      ph_.nnz_eff = host_tot_[kTotScratch];
     }
-    d_colptr = dcp;
-    d_rows = drw;
-    d_dims = ddm;
+    ph_.rows = drw;
     d.colptr = dcp;
     d.dims = ddm;
-  } else if ((p.model & 1) && maxdim >= 1) {
-    rc = guided_clear(d, n, maxdim, d_dims, d_colptr, d_rows, p, &hd, S);
-    if (rc) return rc;
+  } else if ((p.model & 1) && ph_.maxdim >= 1) {
+    if ((rc = guided_clear(d, n, ph_.maxdim, d_dims, d_colptr, d_rows, p, &ph_.hd, S))) return rc;
   }
+  if ((rc = prepare_store())) return rc;
+  ph_.T1 = std::chrono::steady_clock::now();
+  S->t_init_ms = std::chrono::duration<double, std::milli>(ph_.T1 - ph_.T0).count();
+  ph_.ready = true;
+  if (st) *st = *S;
+  return SSQ_OK;
+}
 
-  for (int attempt = 0;; attempt++) {
-    if ((rc = ensure(b_pool_, (size_t)(nnz_eff + arena_cap), &d.pool))) return rc;
-    d.arena_base = nnz_eff;
-    d.arena_cap = arena_cap;
-    DCHK(dcopy(d.pool, d_rows, sizeof(int32_t) * std::max<int64_t>(nnz_eff, 0)));
-    DCHK(dfill(d.ctr, 0, cw::kWords * sizeof(uint32_t)));
-    const int g = grid_for(n);
-    hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d, 1);
-    if (dual)
-      hipLaunchKernelGGL(k_dual_init_from_primal, dim3(g), dim3(256), 0, stream_, n, primal_colptr, primal_rows,
-                         (const uint8_t*)primal_removed, d.leftone, d.flag + n);
-    else
-      hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
-    LoopPlan L;
-    L.U = U;
-    if ((rc = init_lists(d, maxdim, (const uint8_t*)hd, dual, p, &L, S))) return rc;
-    auto T1 = std::chrono::steady_clock::now();
-    S->t_init_ms = std::chrono::duration<double, std::milli>(T1 - T0).count();
+// The column store from the matrix (rows, then an arena of ph_.arena_cap entries), column
+// setup, the init passes and the pending lists.  Run again, with a larger arena, when the page
+// loop exhausts the store.
+int Engine::prepare_store() {
+  Dev& d = ph_.d;
+  const int64_t n = ph_.n;
+  int rc;
+  if ((rc = ensure(b_pool_, (size_t)(ph_.nnz_eff + ph_.arena_cap), &d.pool))) return rc;
+  d.arena_base = ph_.nnz_eff;
+  d.arena_cap = ph_.arena_cap;
+  DCHK(dcopy(d.pool, ph_.rows, sizeof(int32_t) * std::max<int64_t>(ph_.nnz_eff, 0)));
+  DCHK(dfill(d.ctr, 0, cw::kWords * sizeof(uint32_t)));
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_setup, dim3(g), dim3(256), 0, stream_, d, 1);
+  if (ph_.dual)
+    hipLaunchKernelGGL(k_dual_init_from_primal, dim3(g), dim3(256), 0, stream_, n, ph_.primal_colptr, ph_.primal_rows,
+                       (const uint8_t*)ph_.primal_removed, d.leftone, d.flag + n);
+  else
+    hipLaunchKernelGGL(k_leftone, dim3(g), dim3(256), 0, stream_, d);
+  ph_.L = LoopPlan{};
+  ph_.L.U = ph_.U;
+  return init_lists(d, ph_.maxdim, (const uint8_t*)ph_.hd, ph_.dual, ph_.p, &ph_.L, &ph_.S);
+}
 
-    bool oom = false;
-    if ((rc = page_loop(d, p, maxdim, L, S, &oom))) return rc;
-    if (oom) {
-      const int64_t budget = (int64_t)1 << 36;  // 256 GiB of int32 entries at most
-      if (arena_cap * 2 > budget || attempt > 8) {
-        err_ = "column store exhausted";
-        return SSQ_ENOMEM;
-      }
-      arena_cap *= 2;
-      S->n_pages = S->n_rounds = 0;
-      continue;  // rerun from the matrix with a larger arena
-    }
-    // pairs in row order, essential cells; one round trip at the end
-    if (!dual) {
-      rc = compact(n, 1, KeyPivot{d.piv}, EmitPair{d.piv, d_pairs}, nullptr, kTotPairs);
-      if (rc) return rc;
-    } else {  // dual pair (b', d') -> (n-1-d', n-1-b'), sorted by the new birth
-      int32_t* death_of = (int32_t*)d.leftone;
-      DCHK(dfill(death_of, 0xff, sizeof(int32_t) * n));
-      hipLaunchKernelGGL(k_dual_pairs_fill, dim3(grid_for(n)), dim3(256), 0, stream_, n, d.piv, death_of);
-      rc = compact(n, 1, KeyDeathOf{death_of}, EmitDeathOf{death_of, d_pairs}, nullptr, kTotPairs);
-      if (rc) return rc;
-    }
-    if (d_essential && ne_out) {
-      int64_t* ess_dst = d_essential;
-      if (dual && (rc = ensure(b_keys_, (size_t)n, &ess_dst))) return rc;
-      rc = compact(n, 1, KeyEssential{d.piv, d.removed, d.state, d.low}, EmitI64{ess_dst}, nullptr, kTotEss);
-      if (rc) return rc;
-      if (dual)
-        hipLaunchKernelGGL(k_dual_essential, dim3(grid_for(n)), dim3(256), 0, stream_, tot_ + kTotEss, ess_dst, n,
-                           d_essential);
-    }
-    rc = sync_all();
-    if (rc) return rc;
-    *np_out = host_tot_[kTotPairs];
-    if (d_essential && ne_out) *ne_out = host_tot_[kTotEss];
-    fill_stats(S, nnz_eff, arena_cap);
-    auto T2 = std::chrono::steady_clock::now();
-    S->t_reduce_ms = std::chrono::duration<double, std::milli>(T2 - T1).count();
-    S->t_total_ms = std::chrono::duration<double, std::milli>(T2 - T0).count();
+// Phase two (specseq.h:592-1060 GPU_Compute, 1182-1193 GetPersistencePairs): the page loop,
+// then the pairs in row order and the essential cells; one round trip at the end.
+int Engine::compute_phase(int64_t* d_pairs, int64_t* np_out, ssq_stats* st, int64_t* d_essential, int64_t* ne_out) {
+  err_.clear();
+  if (!ph_.ready) {
+    err_ = "no matrix initialized on this handle";
+    return SSQ_EINVAL;
+  }
+  if (!np_out || (ph_.n > 0 && !d_pairs)) {
+    err_ = "invalid arguments";
+    return SSQ_EINVAL;
+  }
+  ph_.ready = false;  // the state is consumed (and changed) by the page loop
+  HIPCHK(hipSetDevice(device_));
+  *np_out = 0;
+  if (ne_out) *ne_out = 0;
+  ssq_stats* S = &ph_.S;
+  const int64_t n = ph_.n;
+  if (n == 0) {
+    if (st) *st = *S;
     return SSQ_OK;
   }
+  const auto Tc = std::chrono::steady_clock::now();
+  Dev& d = ph_.d;
+  int rc;
+  for (int attempt = 0;; attempt++) {
+    bool oom = false;
+    if ((rc = page_loop(d, ph_.p, ph_.maxdim, ph_.L, S, &oom))) return rc;
+    if (!oom) break;
+    const int64_t budget = (int64_t)1 << 36;  // 256 GiB of int32 entries at most
+    if (ph_.arena_cap * 2 > budget || attempt > 8) {
+      err_ = "column store exhausted";
+      return SSQ_ENOMEM;
+    }
+    ph_.arena_cap *= 2;
+    S->n_pages = S->n_rounds = 0;
+    if ((rc = prepare_store())) return rc;  // rerun from the matrix with a larger arena
+  }
+  if (!ph_.dual) {
+    if ((rc = compact(n, 1, KeyPivot{d.piv}, EmitPair{d.piv, d_pairs}, nullptr, kTotPairs))) return rc;
+  } else {  // dual pair (b', d') -> (n-1-d', n-1-b'), sorted by the new birth
+    int32_t* death_of = (int32_t*)d.leftone;
+    DCHK(dfill(death_of, 0xff, sizeof(int32_t) * n));
+    hipLaunchKernelGGL(k_dual_pairs_fill, dim3(grid_for(n)), dim3(256), 0, stream_, n, d.piv, death_of);
+    if ((rc = compact(n, 1, KeyDeathOf{death_of}, EmitDeathOf{death_of, d_pairs}, nullptr, kTotPairs))) return rc;
+  }
+  if (d_essential && ne_out) {
+    int64_t* ess_dst = d_essential;
+    if (ph_.dual && (rc = ensure(b_keys_, (size_t)n, &ess_dst))) return rc;
+    rc = compact(n, 1, KeyEssential{d.piv, d.removed, d.state, d.low}, EmitI64{ess_dst}, nullptr, kTotEss);
+    if (rc) return rc;
+    if (ph_.dual)
+      hipLaunchKernelGGL(k_dual_essential, dim3(grid_for(n)), dim3(256), 0, stream_, tot_ + kTotEss, ess_dst, n,
+                         d_essential);
+  }
+  if ((rc = sync_all())) return rc;
+  *np_out = host_tot_[kTotPairs];
+  if (d_essential && ne_out) *ne_out = host_tot_[kTotEss];
+  fill_stats(S, ph_.nnz_eff, ph_.arena_cap);
+  const auto T2 = std::chrono::steady_clock::now();
+  S->t_reduce_ms = std::chrono::duration<double, std::milli>(T2 - Tc).count();
+  S->t_total_ms = S->t_init_ms + S->t_reduce_ms;
+  if (st) *st = *S;
+  return SSQ_OK;
 }
 
 // Counters of the last page loop (host_ctr_ must hold the device counters).
@@ -3048,6 +3112,7 @@ void Engine::fill_stats(ssq_stats* S, int64_t nnz, int64_t arena_cap) {
 int Engine::prebuilt_begin(int64_t n, int64_t ncols, int64_t nnz, int64_t arena, const int64_t* colptr,
                            const int32_t* dims, int32_t maxdim, PrebuiltBufs* out) {
   err_.clear();
+  ph_.ready = false;  // the buffers are reused: a matrix left by init_phase is discarded
   if (n < 1 || ncols < 1 || ncols > n || n >= 0x7fffffffLL || nnz < 0 || arena < 0 || !colptr || !dims || !out ||
       maxdim < 1 || maxdim >= kMaxDims) {
     err_ = "invalid prebuilt matrix";
@@ -3139,6 +3204,7 @@ struct Handle {
   std::vector<std::unique_ptr<Engine>> xeng;
   std::vector<RipsCo*> xrc;
   int rips_streams = 0;  // 0: SSQ_RIPS_STREAMS or 4
+  int64_t init_cells = -1;  // cells of the matrix ssq_init left on the engine (-1: none)
   DevBuf pts;
   explicit Handle(int dev) : eng(dev) {}
   ~Handle() {
@@ -3223,11 +3289,18 @@ static int split_pairs(Handle* H, const int64_t* dp, int64_t np, const int64_t* 
 // Host work is O(count) checks plus one staged copy; the per-nonzero validation runs on the
 // device before any kernel indexes with the input.  outs[i] receives block i's pairs in its
 // own indices.
-static int reduce_blocks(Handle* H, const ssq_matrix* const* mats, int count, const ssq_params& p, ssq_pairs* outs,
-                         ssq_stats* st) {
+// The matrices into the handle's device buffers, block-diagonally (checked on the device:
+// rows in range and strictly increasing per column), shifted to global indices.
+struct Uploaded {
+  std::vector<int64_t> base;  // first cell of each block, then the total
+  int64_t N = 0, NNZ = 0;
+  int64_t* colptr = nullptr;
+  int32_t* rows = nullptr;
+  int32_t* dims = nullptr;
+};
+static int upload_blocks(Handle* H, const ssq_matrix* const* mats, int count, Uploaded* U) {
   std::vector<int64_t> base((size_t)count + 1, 0), nzb((size_t)count + 1, 0);
   for (int i = 0; i < count; i++) {
-    std::memset(&outs[i], 0, sizeof(ssq_pairs));
     const ssq_matrix* m = mats[i];
     if (!m || m->n < 0 || m->nnz < 0 || (m->n > 0 && (!m->col_ptr || !m->dims)) || (m->nnz > 0 && !m->rows)) {
       H->err = "invalid matrix " + std::to_string(i);
@@ -3307,15 +3380,30 @@ static int reduce_blocks(Handle* H, const ssq_matrix* const* mats, int count, co
       if (hipGetLastError() != hipSuccess) return SSQ_EDEVICE;
     }
   }
+  U->base = std::move(base);
+  U->N = N;
+  U->NNZ = NNZ;
+  U->colptr = dc;
+  U->rows = dr;
+  U->dims = dd;
+  return SSQ_OK;
+}
+
+static int reduce_blocks(Handle* H, const ssq_matrix* const* mats, int count, const ssq_params& p, ssq_pairs* outs,
+                         ssq_stats* st) {
+  for (int i = 0; i < count; i++) std::memset(&outs[i], 0, sizeof(ssq_pairs));
+  Uploaded U;
+  int rc = upload_blocks(H, mats, count, &U);
+  if (rc) return rc;
   int64_t np = 0, ne = 0;
   int64_t* dp = (int64_t*)H->out_pairs.p;
   int64_t* de = (int64_t*)H->out_ess.p;
-  rc = H->eng.reduce_device(N, NNZ, dc, dr, dd, p, dp, &np, st, de, &ne);
+  rc = H->eng.reduce_device(U.N, U.NNZ, U.colptr, U.rows, U.dims, p, dp, &np, st, de, &ne);
   if (rc) {
     H->err = H->eng.error();
     return rc;
   }
-  return split_pairs(H, dp, np, de, ne, base.data(), count, outs);
+  return split_pairs(H, dp, np, de, ne, U.base.data(), count, outs);
 }
 
 // Copy the batch's pairs and essentials to the host and give each block its own run, in its
@@ -3414,6 +3502,47 @@ extern "C" int ssq_reduce(ssq_handle* h, const ssq_matrix* m, const ssq_params* 
   if (hipSetDevice(h->h->eng.device()) != hipSuccess) return SSQ_EDEVICE;
   const ssq_matrix* one[1] = {m};
   return reduce_blocks(h->h, one, 1, *p, out, stats);
+}
+
+extern "C" int ssq_init(ssq_handle* h, const ssq_matrix* m, const ssq_params* p, ssq_stats* stats) {
+  if (!h || !p || !m) return SSQ_EINVAL;
+  Handle* H = h->h;
+  H->err.clear();
+  H->init_cells = -1;
+  if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  const ssq_matrix* one[1] = {m};
+  Uploaded U;
+  int rc = upload_blocks(H, one, 1, &U);
+  if (rc) return rc;
+  if ((rc = H->eng.init_phase(U.N, U.NNZ, U.colptr, U.rows, U.dims, *p, stats))) {
+    H->err = H->eng.error();
+    return rc;
+  }
+  H->init_cells = U.N;
+  return SSQ_OK;
+}
+
+extern "C" int ssq_compute(ssq_handle* h, ssq_pairs* out, ssq_stats* stats) {
+  if (!h || !out) return SSQ_EINVAL;
+  Handle* H = h->h;
+  std::memset(out, 0, sizeof(*out));
+  H->err.clear();
+  if (H->init_cells < 0) {
+    H->err = "no matrix initialized on this handle (ssq_init first)";
+    return SSQ_EINVAL;
+  }
+  const int64_t base[2] = {0, H->init_cells};
+  H->init_cells = -1;
+  if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  int64_t np = 0, ne = 0;
+  int64_t* dp = (int64_t*)H->out_pairs.p;
+  int64_t* de = (int64_t*)H->out_ess.p;
+  int rc = H->eng.compute_phase(dp, &np, stats, de, &ne);
+  if (rc) {
+    H->err = H->eng.error();
+    return rc;
+  }
+  return split_pairs(H, dp, np, de, ne, base, 1, out);
 }
 
 extern "C" int ssq_reduce_device(ssq_handle* h, int64_t n, int64_t nnz, const int64_t* d_col_ptr,

@@ -1,9 +1,10 @@
 """Host-side mirror of the reference engine class SpecSeqPlusPlus (specseq.h:63-1194).
 
 Same constructor arguments and call sequence: construct from per-column boundaries and
-dimensions, GPU_Init(), GPU_Compute(), GetPersistencePairs().  The work is done by libssq;
-GPU_Init uploads and runs the pre-passes, GPU_Compute runs the page loop (both happen inside
-one ssq_reduce call, so GPU_Init only validates and GPU_Compute returns the call's time).
+dimensions, GPU_Init(), GPU_Compute(), GetPersistencePairs().  The work is done by libssq in the
+reference's two phases: GPU_Init (ssq_init: upload, edge collapse, high-dim clearing or the
+anti-transpose, init passes, pending lists) and GPU_Compute (ssq_compute: the page loop and the
+pairs).  Each returns its phase's seconds, as the reference's do (main.cu:337-348).
 """
 import numpy as np
 
@@ -28,22 +29,37 @@ class SpecSeqPlusPlus:
                              queue_size=int(queue_size), level1=int(lc[0]), level2=int(lc[1]))
         self._handle = Handle(device)
         self._pairs = None
+        self._essential = None
         self._stats = None
+        self._initialized = False
         self.DataFileName = ""
 
     def setDataFileName(self, name):
         self.DataFileName = name
 
     def GPU_Init(self):
-        return 0.0
+        """specseq.h:513-590; seconds of the phase."""
+        self._stats = self._handle.init(self.matrix, self.params)
+        self._initialized = True
+        return self._stats["t_init_ms"] / 1000.0
 
     def GPU_Compute(self):
-        pairs, _ess, st = self._handle.reduce(self.matrix, self.params, stats=True)
-        self._pairs = pairs
-        self._stats = st
-        return st["t_total_ms"] / 1000.0
+        """specseq.h:592-1060; seconds of the phase.  Runs GPU_Init first if it has not run."""
+        if not self._initialized:
+            self.GPU_Init()
+        pairs, ess, st = self._handle.compute(stats=True)
+        self._initialized = False
+        self._pairs, self._essential, self._stats = pairs, ess, st
+        return st["t_reduce_ms"] / 1000.0
 
     def GetPersistencePairs(self):
+        """specseq.h:1182-1193: (birth, death) pairs in birth order."""
         if self._pairs is None:
             self.GPU_Compute()
         return [(int(b), int(d)) for b, d in self._pairs]
+
+    def GetEssential(self):
+        """Cells that stay unpaired (the output's essential classes)."""
+        if self._pairs is None:
+            self.GPU_Compute()
+        return [int(c) for c in self._essential]

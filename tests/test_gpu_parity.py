@@ -402,3 +402,37 @@ def test_dual_sort_nine_bit_digits(h):
     outs = h.batch_reduce(mats, S.params(dualize=1, ss_block_size=1 << 30))
     for m, (pairs, ess) in zip(mats, outs):
         assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))
+
+
+def test_two_phases_equal_one_call(h):
+    """ssq_init + ssq_compute (the reference's GPU_Init / GPU_Compute) give ssq_reduce's pairs in
+    every mode; compute without an initialized matrix, or after another reduction discarded it,
+    is refused."""
+    mo = c1_matrix()
+    m = as_product(mo)
+    for kw in (dict(), dict(dualize=1), dict(model=1), dict(model=3), dict(level1=1, level2=2, ss_block_size=64)):
+        p = S.params(**kw)
+        st_i = h.init(m, p)
+        pairs, ess, st_c = h.compute(stats=True)
+        ref, ref_ess = h.reduce(m, p)
+        assert np.array_equal(pairs, ref) and np.array_equal(ess, ref_ess), kw
+        assert st_i["t_init_ms"] > 0 and st_c["t_reduce_ms"] > 0 and st_c["n_pending"] == st_i["n_pending"]
+    with pytest.raises(_lib.SSQError):
+        h.compute()
+    h.init(m)
+    h.reduce(m)
+    with pytest.raises(_lib.SSQError):
+        h.compute()
+
+
+def test_reference_class_mirror():
+    from specseqplusplus_amd.specseq import SpecSeqPlusPlus
+    mo = c1_matrix()
+    cols = [mo.rows[mo.colptr[c]:mo.colptr[c + 1]].tolist() for c in range(mo.n)]
+    ss = SpecSeqPlusPlus(cols, mo.dims.tolist(), [0, 1000, 2000, 0], dualize=0, model=1)
+    t_init = ss.GPU_Init()
+    t_comp = ss.GPU_Compute()
+    assert t_init > 0 and t_comp > 0
+    got = np.array(ss.GetPersistencePairs(), dtype=np.int64).reshape(-1, 2)
+    assert np.array_equal(got, O.twist_pairs(mo))
+    assert 2 * len(got) + len(ss.GetEssential()) == mo.n

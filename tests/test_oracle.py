@@ -96,6 +96,48 @@ def test_collapse_subcomplex_invariants():
     assert ess_full == ess_sub0
 
 
+def _low_betti(m, pairs, skip):
+    paired = np.zeros(m.n, bool)
+    paired[pairs[:, 0]] = True
+    paired[pairs[:, 1]] = True
+    alive = ~paired & (skip == 0)
+    return [int((alive & (m.dims == k)).sum()) for k in (0, 1)]
+
+
+def _collapse_cases():
+    rng = np.random.default_rng(1)
+    for _ in range(40):
+        n = int(rng.integers(20, 90))
+        yield C.vr_matrix(rng.random((n, 3)), float(rng.uniform(0.15, 0.5)), 2)
+
+
+def test_collapse_keeps_beta0_and_beta1():
+    """A strong collapse keeps the flag complex's homotopy type: the kept sub-complex has the
+    full complex's beta_0 and beta_1 (its 2-skeleton's beta_2 may differ)."""
+    for m in _collapse_cases():
+        skip = O.collapse_skip(m)
+        assert _low_betti(m, O.twist_pairs(m, skip=skip), skip) == \
+            _low_betti(m, O.twist_pairs(m), np.zeros(m.n, np.uint8))
+
+
+def test_collapse_betti_check_catches_over_removal():
+    """Planted mistake: one more kept edge (the last kept one) removed with its cofaces changes
+    beta_0 or beta_1 in some of the same cases."""
+    caught = 0
+    for m in _collapse_cases():
+        skip = O.collapse_skip(m).copy()
+        kept_edges = np.where((m.dims == 1) & (skip == 0))[0]
+        if len(kept_edges) < 2:
+            continue
+        skip[kept_edges[-1]] = 1
+        for c in np.where(m.dims == 2)[0]:  # its cofaces too
+            if skip[m.rows[m.colptr[c]:m.colptr[c + 1]]].any():
+                skip[c] = 1
+        caught += _low_betti(m, O.twist_pairs(m, skip=skip), skip) != \
+            _low_betti(m, O.twist_pairs(m), np.zeros(m.n, np.uint8))
+    assert caught >= 10
+
+
 def test_planted_mistakes_are_caught():
     """A wrong pairing must fail the independent checks used above."""
     m = C.closure_matrix(C.TORUS_7, 2)

@@ -292,7 +292,7 @@ def served_leg(args):
         picks = rng.integers(0, 1 << 30, args.served)
         gaps = rng.exponential(1.0 / qps, args.served)
         lat = []
-        with Server(streams=args.server_streams, max_batch=64, max_batch_cells=args.server_cells,
+        with Server(streams=args.server_streams, max_batch=args.server_batch, max_batch_cells=args.server_cells,
                     p=S.params(**SERVED_MATRIX_PARAMS), points_p=S.params(**SERVED_PARAMS)) as srv:
             tickets = []
             ts = time.perf_counter()
@@ -408,6 +408,7 @@ def main():
     ap.add_argument("--pool-mats", type=int, default=24)
     ap.add_argument("--server-streams", type=int, default=4)
     ap.add_argument("--server-cells", type=int, default=2_000_000)
+    ap.add_argument("--server-batch", type=int, default=64, help="requests per served batch at most")
     ap.add_argument("--cpu-seconds", type=float, default=45.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the C2 sub-record")

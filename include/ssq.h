@@ -49,6 +49,13 @@ typedef struct ssq_params {
   int32_t queue_size;    /* additions in flight per round (one wavefront each)          */
   int32_t level1;        /* level-1 slot capacity (int32 entries)                        */
   int32_t level2;        /* level-2 slot capacity (int32 entries)                        */
+  /* Workgroup shape of the column additions: no reference counterpart (its shapes are
+   * compile-time, bupt.h:48-71).  0 selects the engine's default; the pairs never depend on
+   * them; the autotuner picks them per size class. */
+  int32_t add_waves;       /* wavefronts per short-addition workgroup: 2, 4 or 8 (4)          */
+  int32_t long_grid;       /* workgroups of the long-column kernel, <= 4096 (2 per CU)        */
+  int32_t short_max;       /* largest |a|+|b| a wavefront merges in LDS, 64..1024 (1024)      */
+  int32_t rounds_per_sync; /* add rounds between two reads of the worklist counts, 1..8 (3)   */
 } ssq_params;
 
 /* Persistence pairs, sorted by birth; optional essential (never-paired) cells. */

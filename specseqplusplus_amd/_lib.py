@@ -22,7 +22,8 @@ class ssq_matrix(ctypes.Structure):
 class ssq_params(ctypes.Structure):
     _fields_ = [("dualize", ctypes.c_int32), ("model", ctypes.c_int32), ("max_iteration", ctypes.c_int32),
                 ("ss_block_size", ctypes.c_int32), ("growth", ctypes.c_float), ("queue_size", ctypes.c_int32),
-                ("level1", ctypes.c_int32), ("level2", ctypes.c_int32)]
+                ("level1", ctypes.c_int32), ("level2", ctypes.c_int32), ("add_waves", ctypes.c_int32),
+                ("long_grid", ctypes.c_int32), ("short_max", ctypes.c_int32), ("rounds_per_sync", ctypes.c_int32)]
 
 
 class ssq_pairs(ctypes.Structure):

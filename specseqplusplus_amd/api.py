@@ -41,10 +41,13 @@ class Matrix:
 
 
 def params(dualize=0, model=0, max_iteration=10, ss_block_size=102400, growth=1.05, queue_size=65536,
-           level1=1000, level2=2000):
+           level1=1000, level2=2000, add_waves=0, long_grid=0, short_max=0, rounds_per_sync=0):
+    """ssq_params: the reference's eight parameters, then the additions' workgroup shape (0 =
+    the engine's default)."""
     p = L.ssq_params()
     p.dualize, p.model, p.max_iteration, p.ss_block_size = dualize, model, max_iteration, ss_block_size
     p.growth, p.queue_size, p.level1, p.level2 = growth, queue_size, level1, level2
+    p.add_waves, p.long_grid, p.short_max, p.rounds_per_sync = add_waves, long_grid, short_max, rounds_per_sync
     return p
 
 

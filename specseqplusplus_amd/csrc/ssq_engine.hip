@@ -48,7 +48,7 @@ namespace ssq {
 // ------------------------------------------------------------------------------------
 // device state
 // ------------------------------------------------------------------------------------
-constexpr int kAddWaves = 4;           // wavefronts per add workgroup
+constexpr int kAddWavesDefault = 4;    // wavefronts per short-addition workgroup (2, 4 or 8)
 #ifndef SSQ_KTILE
 #define SSQ_KTILE 1024
 #endif
@@ -577,6 +577,7 @@ __device__ int wave_tiled_xor(const int32_t* A, int32_t a_len, const int32_t* B,
 // smaller column and its low stays in the row band (kernel.h:814-1065 does one addition per
 // warp, or a block-wide chase).  The running column stays in LDS between additions while it
 // and the pivot fit; longer ones merge tile by tile into a fresh store slot.
+template <int kAddWaves>
 __global__ void __launch_bounds__(kWave * kAddWaves) k_add(Dev d, int r, int32_t minrow) {
   __shared__ int32_t lds[kAddWaves][3][kTileBuf];
   const int in = r & 1, nx = (r + 1) & 1;
@@ -2142,6 +2143,13 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// The additions' workgroup shape in the parameters (0 = default) is one the engine supports.
+static bool shape_ok(const ssq_params& p) {
+  return (p.add_waves == 0 || p.add_waves == 2 || p.add_waves == 4 || p.add_waves == 8) && p.long_grid >= 0 &&
+         p.long_grid <= 4096 && (p.short_max == 0 || (p.short_max >= 64 && p.short_max <= kTile)) &&
+         p.rounds_per_sync >= 0 && p.rounds_per_sync <= 8;
+}
+
 class Engine : public PrebuiltTarget {
  public:
   explicit Engine(int device) : device_(device) {}
@@ -2168,8 +2176,12 @@ class Engine : public PrebuiltTarget {
     cus_ = prop.multiProcessorCount;
     // add waves resident per CU: the LDS staging buffers, not registers, set the limit
     int nb = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_add, kWave * kAddWaves, 0));
-    add_waves_per_cu_ = std::max(1, nb) * kAddWaves;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_add<2>, kWave * 2, 0));
+    add_waves_per_cu_[0] = std::max(1, nb) * 2;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_add<4>, kWave * 4, 0));
+    add_waves_per_cu_[1] = std::max(1, nb) * 4;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_add<8>, kWave * 8, 0));
+    add_waves_per_cu_[2] = std::max(1, nb) * 8;
     const char* kt = std::getenv("SSQ_KERNEL_TIMING");
     timing_ = kt && kt[0] == '1';
     const char* tr = std::getenv("SSQ_TRACE_ROUNDS");  // per-round diagnostics on stderr
@@ -2392,7 +2404,7 @@ class Engine : public PrebuiltTarget {
 
   int device_ = 0;
   int cus_ = 256;
-  int add_waves_per_cu_ = 12;
+  int add_waves_per_cu_[3] = {12, 12, 12};  // resident add wavefronts per CU with 2, 4, 8 per workgroup
   hipStream_t stream_ = nullptr;
   uint32_t* host_ctr_ = nullptr;
   std::string err_;
@@ -2799,10 +2811,14 @@ int Engine::page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPla
   if ((rc = ensure_gbits(&d))) return rc;
   const int64_t wcap = std::max<int64_t>(upend, 1);
   for (int k = 0; k < 6; k++) d.work[k] = W0 + k * wcap;
-  const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * add_waves_per_cu_);
-  const int add_grid = std::max(1, (add_waves + kAddWaves - 1) / kAddWaves);
-  const int rounds_per_sync = rounds_per_sync_;
-  const int long_grid = cus_ * 2;
+  // workgroup shape: the parameters' choice, else the defaults
+  const int wpg = p.add_waves > 0 ? p.add_waves : kAddWavesDefault;  // 2, 4 or 8 (checked)
+  const int wsel = wpg == 2 ? 0 : wpg == 4 ? 1 : 2;
+  const int add_waves = (int)std::min<int64_t>(p.queue_size, (int64_t)cus_ * add_waves_per_cu_[wsel]);
+  const int add_grid = std::max(1, (add_waves + wpg - 1) / wpg);
+  const int rounds_per_sync = p.rounds_per_sync > 0 ? p.rounds_per_sync : rounds_per_sync_;
+  const int long_grid = p.long_grid > 0 ? p.long_grid : cus_ * 2;
+  if (p.short_max > 0) d.short_max = std::min(p.short_max, (int32_t)kTile);
   bool oom = false;
   *oom_out = false;
   for (int32_t dim = maxdim; dim >= 1 && !oom; dim--) {
@@ -2827,7 +2843,12 @@ int Engine::page_loop(Dev& d, const ssq_params& p, int32_t maxdim, const LoopPla
             hipLaunchKernelGGL(k_claim, dim3(cg), dim3(256), 0, stream_, d, r);
             hipLaunchKernelGGL(k_pivmeta, dim3(cg), dim3(256), 0, stream_, d, r);
             if (timing_) HIPCHK(hipEventRecord(ev_[4 * t], stream_));
-            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(kWave * kAddWaves), 0, stream_, d, r, fl);
+            if (wpg == 2)
+              hipLaunchKernelGGL(k_add<2>, dim3(add_grid), dim3(kWave * 2), 0, stream_, d, r, fl);
+            else if (wpg == 4)
+              hipLaunchKernelGGL(k_add<4>, dim3(add_grid), dim3(kWave * 4), 0, stream_, d, r, fl);
+            else
+              hipLaunchKernelGGL(k_add<8>, dim3(add_grid), dim3(kWave * 8), 0, stream_, d, r, fl);
             if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 1], stream_));
             hipLaunchKernelGGL(k_add_long, dim3(long_grid), dim3(kLongThreads), 0, stream_, d, r, fl);
             if (timing_) HIPCHK(hipEventRecord(ev_[4 * t + 2], stream_));
@@ -2911,7 +2932,7 @@ int Engine::init_phase(int64_t n, int64_t nnz, const int64_t* d_colptr, const in
     return SSQ_EINVAL;
   }
   if (p.ss_block_size < 1 || !(p.growth >= 1.0f) || p.queue_size < 1 || p.level1 < 1 || p.level2 < 1 ||
-      p.max_iteration < 0 || (p.model & ~3) || (p.dualize & ~1)) {
+      p.max_iteration < 0 || (p.model & ~3) || (p.dualize & ~1) || !shape_ok(p)) {
     err_ = "invalid parameters";
     return SSQ_EINVAL;
   }
@@ -3149,7 +3170,8 @@ int Engine::prebuilt_begin(int64_t n, int64_t ncols, int64_t nnz, int64_t arena,
 
 int Engine::prebuilt_reduce(const ssq_params& p, ssq_stats* st, bool* oom) {
   *oom = false;
-  if (p.ss_block_size < 1 || !(p.growth >= 1.0f) || p.queue_size < 1 || p.level1 < 1 || p.level2 < 1) {
+  if (p.ss_block_size < 1 || !(p.growth >= 1.0f) || p.queue_size < 1 || p.level1 < 1 || p.level2 < 1 ||
+      !shape_ok(p)) {
     err_ = "invalid parameters";
     return SSQ_EINVAL;
   }

@@ -210,4 +210,8 @@ extern "C" void ssq_default_params(ssq_params* p) {
   p->queue_size = 65536;
   p->level1 = 1000;
   p->level2 = 2000;
+  p->add_waves = 0;
+  p->long_grid = 0;
+  p->short_max = 0;
+  p->rounds_per_sync = 0;
 }

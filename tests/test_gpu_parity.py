@@ -436,3 +436,45 @@ def test_reference_class_mirror():
     got = np.array(ss.GetPersistencePairs(), dtype=np.int64).reshape(-1, 2)
     assert np.array_equal(got, O.twist_pairs(mo))
     assert 2 * len(got) + len(ss.GetEssential()) == mo.n
+
+
+SHAPE_VARIANTS = [dict(add_waves=2), dict(add_waves=8), dict(short_max=64), dict(short_max=256),
+                  dict(long_grid=1), dict(long_grid=4096), dict(rounds_per_sync=1), dict(rounds_per_sync=8),
+                  dict(add_waves=2, short_max=64, long_grid=7, rounds_per_sync=5)]
+
+
+@pytest.mark.parametrize("shape", SHAPE_VARIANTS, ids=lambda s: ",".join(f"{k}={v}" for k, v in s.items()))
+def test_workgroup_shapes_are_exact(h, shape):
+    """The additions' workgroup shape never changes the pairs: homology and cohomology of C1,
+    and the batched Rips path (where long columns appear)."""
+    mo = c1_matrix()
+    for base in (dict(), dict(dualize=1), dict(model=1, level1=1, level2=2)):
+        pairs, ess = h.reduce(as_product(mo), S.params(**base, **shape))
+        assert np.array_equal(pairs, O.twist_pairs(mo)), (base, shape)
+    rng = np.random.default_rng(3)
+    clouds = [(rng.random((n, 3)).astype(np.float32), 0.45) for n in (120, 300, 500)]
+    outs = h.batch_reduce_points(clouds, 2, S.params(dualize=1, ss_block_size=1 << 30, **shape))
+    for (X, r), (pairs, _ess) in zip(clouds, outs):
+        m = S.vr_build(X, r, 2)
+        assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims))), shape
+
+
+def test_shape_out_of_range_is_refused(h):
+    mo = c1_matrix()
+    for bad in (dict(add_waves=3), dict(short_max=32), dict(short_max=2048), dict(long_grid=-1),
+                dict(rounds_per_sync=9)):
+        with pytest.raises(_lib.SSQError):
+            h.reduce(as_product(mo), S.params(**bad))
+
+
+def test_autotuner_chooses_params_and_shape(h):
+    from specseqplusplus_amd.autotune import tune
+    mats = []
+    for seed in range(6):
+        X = np.asarray(C.uniform_points(60 + 20 * seed, 3, 300 + seed), dtype=np.float32)
+        mats.append(S.vr_build(X, 0.3, 2))
+    chosen, table = tune(h, mats, repeats=1)
+    assert len(table) >= 9 and table[0]["params"] == chosen
+    res = h.batch_reduce(mats, S.params(**chosen))
+    for m, (pairs, _ess) in zip(mats, res):
+        assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))

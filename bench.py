@@ -313,15 +313,20 @@ def served_leg(args):
             srv.wait_all()
             te = time.perf_counter()
             exact = True
+            kind_of = []
             for kind, j, tk in tickets:
                 res = srv.poll(tk)
                 lat.append(res[2])
+                kind_of.append(kind)
                 exact &= bool(np.array_equal(res[0], pool_ref[kind][j]))
         lat = np.array(lat)
+        kind_of = np.array(kind_of)
         rate = args.served / (te - ts)
         rec = {"qps_offered": qps, "diagrams_per_s": round(rate, 1),
                "over_plain": round(rate / served["plain_per_request_diagrams_per_s"], 2),
                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "p99_ms_clouds": round(float(np.percentile(lat[kind_of == 0], 99)), 3) if (kind_of == 0).any() else None,
+               "p99_ms_matrices": round(float(np.percentile(lat[kind_of == 1], 99)), 3) if (kind_of == 1).any() else None,
                "bit_exact_vs_oracle": exact}
         served["loads"].append(rec)
         log(f"served {rec}")

@@ -284,8 +284,22 @@ def served_leg(args):
         else:
             h.reduce(pool_mats[picks[k] % len(pool_mats)], pp_mat)
     served["plain_per_request_diagrams_per_s"] = round(n_plain / (time.perf_counter() - t), 1)
+    # the same, with the matrices already in device memory (one resident matrix per call)
+    dms = [h.to_device(m) for m in pool_mats]
+    t = time.perf_counter()
+    for k in range(n_plain):
+        if kinds[k] == 0:
+            X, r = pool_pts[picks[k] % len(pool_pts)]
+            h.batch_reduce_points([(X, r)], 2, pp_pts)
+        else:
+            h.batch_reduce_resident([dms[picks[k] % len(pool_mats)]], pp_mat)
+    served["plain_per_request_resident_diagrams_per_s"] = round(n_plain / (time.perf_counter() - t), 1)
+    for dm in dms:
+        h.free_device_matrix(dm)
     h.close()
-    log(f"served mix, plain one request at a time: {served['plain_per_request_diagrams_per_s']} diagrams/s")
+    log(f"served mix, plain one request at a time: {served['plain_per_request_diagrams_per_s']} diagrams/s "
+        f"(matrices from the host), {served['plain_per_request_resident_diagrams_per_s']} (matrices resident)")
+    served["matrices"] = "registered (resident in HBM)" if args.served_matrices == "registered" else "host"
     for qps in args.qps:
         rng = np.random.default_rng(int(qps))
         kinds = rng.integers(0, 2, args.served)
@@ -294,6 +308,7 @@ def served_leg(args):
         lat = []
         with Server(streams=args.server_streams, max_batch=args.server_batch, max_batch_cells=args.server_cells,
                     p=S.params(**SERVED_MATRIX_PARAMS), points_p=S.params(**SERVED_PARAMS)) as srv:
+            mids = [srv.register_matrix(m) for m in pool_mats] if args.served_matrices == "registered" else None
             tickets = []
             ts = time.perf_counter()
             nxt = ts
@@ -309,9 +324,10 @@ def served_leg(args):
                     X, r = pool_pts[j]
                     tickets.append((0, j, srv.submit_points(X, r, 2)))
                 else:
-                    tickets.append((1, j, srv.submit_matrix(pool_mats[j])))
+                    tickets.append((1, j, srv.submit_registered(mids[j]) if mids else srv.submit_matrix(pool_mats[j])))
             srv.wait_all()
             te = time.perf_counter()
+            sstats = srv.stats()
             exact = True
             kind_of = []
             for kind, j, tk in tickets:
@@ -324,10 +340,11 @@ def served_leg(args):
         rate = args.served / (te - ts)
         rec = {"qps_offered": qps, "diagrams_per_s": round(rate, 1),
                "over_plain": round(rate / served["plain_per_request_diagrams_per_s"], 2),
+               "over_plain_resident": round(rate / served["plain_per_request_resident_diagrams_per_s"], 2),
                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                "p99_ms_clouds": round(float(np.percentile(lat[kind_of == 0], 99)), 3) if (kind_of == 0).any() else None,
                "p99_ms_matrices": round(float(np.percentile(lat[kind_of == 1], 99)), 3) if (kind_of == 1).any() else None,
-               "bit_exact_vs_oracle": exact}
+               "bit_exact_vs_oracle": exact, "batches": sstats}
         served["loads"].append(rec)
         log(f"served {rec}")
     return served
@@ -408,12 +425,14 @@ def main():
     ap.add_argument("--nmax", type=int, default=2048)
     ap.add_argument("--plain", type=int, default=48, help="random clouds timed one call each (plain GPU)")
     ap.add_argument("--served", type=int, default=20000, help="served requests per offered load (0 = skip)")
-    ap.add_argument("--qps", type=float, nargs="+", default=[1000.0, 2000.0, 2500.0, 4000.0])
+    ap.add_argument("--qps", type=float, nargs="+", default=[1000.0, 1800.0, 2000.0, 2500.0])
     ap.add_argument("--pool-clouds", type=int, default=192)
     ap.add_argument("--pool-mats", type=int, default=24)
     ap.add_argument("--server-streams", type=int, default=4)
-    ap.add_argument("--server-cells", type=int, default=2_000_000)
-    ap.add_argument("--server-batch", type=int, default=64, help="requests per served batch at most")
+    ap.add_argument("--server-cells", type=int, default=4_000_000)
+    ap.add_argument("--server-batch", type=int, default=32, help="requests per served batch at most")
+    ap.add_argument("--served-matrices", choices=["registered", "host"], default="registered",
+                    help="matrix requests by the id of a matrix registered in HBM, or as host matrices")
     ap.add_argument("--cpu-seconds", type=float, default=45.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the C2 sub-record")

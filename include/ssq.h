@@ -154,6 +154,23 @@ int ssq_device_synchronize(ssq_handle* h);
 int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, int count,
                      const ssq_params* p, ssq_pairs* outs, ssq_stats* stats);
 
+/* ---- Matrices resident in HBM (new; the reference re-reads its input for every run,
+ * main.cu:322-335).  ssq_matrix_to_device copies a host matrix into device memory of its own,
+ * checked on the device (row range, strictly increasing rows); ssq_device_matrix_free releases
+ * it (hipFree: it waits for the device).  ssq_batch_reduce_resident is ssq_batch_reduce on such
+ * matrices: each is gathered into the batch by one device kernel, with no host staging and no
+ * copy from the host; outs[i] holds matrix i's pairs in its own indices. */
+typedef struct ssq_device_matrix {
+  int64_t n, nnz;
+  const int64_t* col_ptr; /* device memory, n+1 offsets, col_ptr[0] = 0 */
+  const int32_t* rows;    /* device memory, nnz rows                    */
+  const int32_t* dims;    /* device memory, n dimensions                */
+} ssq_device_matrix;
+int ssq_matrix_to_device(ssq_handle* h, const ssq_matrix* m, ssq_device_matrix* out);
+int ssq_device_matrix_free(ssq_handle* h, ssq_device_matrix* m);
+int ssq_batch_reduce_resident(ssq_handle* h, const ssq_device_matrix* mats, int count,
+                              const ssq_params* p, ssq_pairs* outs, ssq_stats* stats);
+
 /* ---- Serving (new): continuous batching request loop on one GPU. -------------------- */
 typedef struct ssq_server ssq_server;
 typedef struct ssq_server_config {
@@ -171,10 +188,26 @@ int ssq_server_create(const ssq_server_config* cfg, ssq_server** out);
 int ssq_server_submit_matrix(ssq_server* s, const ssq_matrix* m, int64_t* ticket);
 int ssq_server_submit_points(ssq_server* s, const float* pts, int64_t npts, int pdim,
                              float r, int maxdim, int64_t* ticket);
+/* Matrices kept resident in HBM for requests that repeat: register checks the matrix on the
+ * host once (offsets, rows strictly increasing and in range) and copies it into device memory
+ * owned by the server; submit_registered queues a request on it (no host copy, no upload);
+ * unregister waits for that matrix's outstanding requests and frees it (hipFree waits for the
+ * device).  Registered requests use cfg.params and are batched with each other. */
+int ssq_server_register_matrix(ssq_server* s, const ssq_matrix* m, int64_t* matrix_id);
+int ssq_server_submit_registered(ssq_server* s, int64_t matrix_id, int64_t* ticket);
+int ssq_server_unregister_matrix(ssq_server* s, int64_t matrix_id);
 /* Non-blocking poll: returns SSQ_OK and fills out (caller frees) when ticket is done,
  * SSQ_EBUSY when still pending.  latency_ms (may be NULL): submit-to-completion. */
 int ssq_server_poll(ssq_server* s, int64_t ticket, ssq_pairs* out, double* latency_ms);
 int ssq_server_wait_all(ssq_server* s);
+/* Counters since the server started, per batch kind (0: host matrices, 1: point clouds,
+ * 2: registered matrices): batches, requests, cells, and the batches' service time (from
+ * admission to published results) summed and at most. */
+typedef struct ssq_server_stats {
+  int64_t batches[3], requests[3], cells[3];
+  double service_ms[3], max_service_ms[3];
+} ssq_server_stats;
+int ssq_server_get_stats(ssq_server* s, ssq_server_stats* out);
 int ssq_server_destroy(ssq_server* s);
 
 /* Host edge collapse (model bit 1): kept[i] = 1 for every edge that GUDHI's sequential

@@ -47,10 +47,20 @@ class ssq_stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ssq_device_matrix(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("col_ptr", ctypes.c_void_p),
+                ("rows", ctypes.c_void_p), ("dims", ctypes.c_void_p)]
+
+
 class ssq_server_config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("streams", ctypes.c_int), ("max_batch", ctypes.c_int),
                 ("max_batch_cells", ctypes.c_int64), ("mem_budget_bytes", ctypes.c_int64), ("params", ssq_params),
                 ("points_params", ssq_params)]
+
+
+class ssq_server_stats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_int64 * 3), ("requests", ctypes.c_int64 * 3), ("cells", ctypes.c_int64 * 3),
+                ("service_ms", ctypes.c_double * 3), ("max_service_ms", ctypes.c_double * 3)]
 
 
 # every symbol declared in include/ssq.h: name -> (restype, argtypes)
@@ -74,6 +84,15 @@ SIGNATURES = {
     "ssq_batch_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(ssq_matrix)), ctypes.c_int,
                                         ctypes.POINTER(ssq_params), ctypes.POINTER(ssq_pairs),
                                         ctypes.POINTER(ssq_stats)]),
+    "ssq_matrix_to_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_matrix),
+                                            ctypes.POINTER(ssq_device_matrix)]),
+    "ssq_device_matrix_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_device_matrix)]),
+    "ssq_batch_reduce_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_device_matrix), ctypes.c_int,
+                                                 ctypes.POINTER(ssq_params), ctypes.POINTER(ssq_pairs),
+                                                 ctypes.POINTER(ssq_stats)]),
+    "ssq_server_register_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_matrix), i64p]),
+    "ssq_server_submit_registered": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, i64p]),
+    "ssq_server_unregister_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "ssq_server_create": (ctypes.c_int, [ctypes.POINTER(ssq_server_config), ctypes.POINTER(ctypes.c_void_p)]),
     "ssq_server_submit_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_matrix), i64p]),
     "ssq_server_submit_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64,
@@ -81,6 +100,7 @@ SIGNATURES = {
     "ssq_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ssq_pairs),
                                        ctypes.POINTER(ctypes.c_double)]),
     "ssq_server_wait_all": (ctypes.c_int, [ctypes.c_void_p]),
+    "ssq_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ssq_server_stats)]),
     "ssq_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "ssq_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "ssq_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

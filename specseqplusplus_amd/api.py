@@ -236,6 +236,28 @@ class Handle:
         """ssq_reduce_device on an uploaded matrix; returns (n_pairs, stats)."""
         return self.reduce_device(dm.n, dm.nnz, dm.colptr.ptr, dm.rows.ptr, dm.dims.ptr, d_pairs.ptr, p)
 
+    def to_device(self, m):
+        """ssq_matrix_to_device: m copied into device memory of its own, checked on the device.
+        Freed by free_device_matrix (or when the handle closes its process)."""
+        dm = L.ssq_device_matrix()
+        ms = m.c_struct()
+        L.check(L.lib().ssq_matrix_to_device(self._h, ctypes.byref(ms), ctypes.byref(dm)), self.error())
+        return dm
+
+    def free_device_matrix(self, dm):
+        L.check(L.lib().ssq_device_matrix_free(self._h, ctypes.byref(dm)), self.error())
+
+    def batch_reduce_resident(self, dms, p=None, stats=False):
+        """ssq_batch_reduce_resident over matrices from to_device: [(pairs, essential), ...]."""
+        p = p or params()
+        arr = (L.ssq_device_matrix * len(dms))(*dms)
+        outs = (L.ssq_pairs * max(len(dms), 1))()
+        st = L.ssq_stats()
+        L.check(L.lib().ssq_batch_reduce_resident(self._h, arr, len(dms), ctypes.byref(p), outs, ctypes.byref(st)),
+                self.error())
+        res = [_pairs_from_c(outs[i]) for i in range(len(dms))]
+        return (res, st.as_dict()) if stats else res
+
     def batch_reduce(self, mats, p=None, stats=False):
         p = p or params()
         structs = [m.c_struct() for m in mats]

@@ -1875,6 +1875,18 @@ __global__ void k_batch_shift(int64_t n, int64_t nnz, int64_t count, const int64
   if (blockIdx.x == 0 && threadIdx.x == 0) colptr[n] = nnz;
 }
 
+// One resident matrix into the batch buffers at cell offset `base` and nonzero offset `nzb`:
+// offsets and rows shifted to global indices.  Entry n of the offsets equals the next block's
+// entry 0, so consecutive launches on one stream write the same value there.
+__global__ void k_gather_block(int64_t n, int64_t nnz, const int64_t* __restrict__ cp,
+                               const int32_t* __restrict__ rows, const int32_t* __restrict__ dims, int64_t base,
+                               int64_t nzb, int64_t* __restrict__ dcp, int32_t* __restrict__ drows,
+                               int32_t* __restrict__ ddims) {
+  GRID_STRIDE(c, n + 1) dcp[c] = cp[c] + nzb;
+  GRID_STRIDE(c, n) ddims[c] = dims[c];
+  GRID_STRIDE(k, nnz) drows[k] = rows[k] + (int32_t)base;
+}
+
 // ------------------------------------------------------------------------------------
 // edge collapse support (model bit 1)
 // ------------------------------------------------------------------------------------
@@ -2202,6 +2214,9 @@ class Engine : public PrebuiltTarget {
   const std::string& error() const { return err_; }
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
+  // byte fill and device-to-device copy on the engine's stream, by kernels
+  int fill(void* dst, int value, size_t bytes) { return dfill(dst, value, bytes); }
+  int copy(void* dst, const void* src, size_t bytes) { return dcopy(dst, src, bytes); }
 
   int reduce_device(int64_t n, int64_t nnz, const int64_t* d_colptr, const int32_t* d_rows,
                     const int32_t* d_dims, const ssq_params& p, int64_t* d_pairs, int64_t* np_out,
@@ -3411,6 +3426,56 @@ static int upload_blocks(Handle* H, const ssq_matrix* const* mats, int count, Up
   return SSQ_OK;
 }
 
+// Matrices already resident in HBM (validated when they were placed there) into the batch
+// buffers: one gather kernel per block, no host staging, no host round trip.
+static int gather_blocks(Handle* H, const ssq_device_matrix* mats, int count, Uploaded* U) {
+  std::vector<int64_t> base((size_t)count + 1, 0), nzb((size_t)count + 1, 0);
+  for (int i = 0; i < count; i++) {
+    const ssq_device_matrix& m = mats[i];
+    if (m.n < 0 || m.nnz < 0 || (m.n > 0 && (!m.col_ptr || !m.dims)) || (m.nnz > 0 && !m.rows) ||
+        (m.n == 0 && m.nnz != 0)) {
+      H->err = "invalid resident matrix " + std::to_string(i);
+      return SSQ_EINVAL;
+    }
+    base[i + 1] = base[i] + m.n;
+    nzb[i + 1] = nzb[i] + m.nnz;
+  }
+  const int64_t N = base[count], NNZ = nzb[count];
+  if (N >= 0x7fffffffLL) {
+    H->err = "more than 2^31 - 1 cells";
+    return SSQ_EINVAL;
+  }
+  int rc;
+  if ((rc = dev_grow(H, H->in_colptr, sizeof(int64_t) * (N + 1))) ||
+      (rc = dev_grow(H, H->in_rows, sizeof(int32_t) * NNZ)) || (rc = dev_grow(H, H->in_dims, sizeof(int32_t) * N)) ||
+      (rc = dev_grow(H, H->out_pairs, sizeof(int64_t) * 2 * N)) || (rc = dev_grow(H, H->out_ess, sizeof(int64_t) * N)))
+    return rc;
+  hipStream_t s = H->eng.stream();
+  int64_t* dc = (int64_t*)H->in_colptr.p;
+  int32_t* dr = (int32_t*)H->in_rows.p;
+  int32_t* dd = (int32_t*)H->in_dims.p;
+  if (N == 0 && (rc = H->eng.fill(dc, 0, sizeof(int64_t)))) return rc;  // colptr[0] = 0
+  for (int i = 0; i < count; i++) {
+    const ssq_device_matrix& m = mats[i];
+    if (m.n == 0) continue;
+    const int64_t work = std::max<int64_t>(m.n + 1, m.nnz);
+    const int grid = (int)std::max<int64_t>(8, std::min<int64_t>((work + 255) / 256, 2048) / 8 * 8);
+    hipLaunchKernelGGL(k_gather_block, dim3(grid), dim3(256), 0, s, m.n, m.nnz, m.col_ptr, m.rows, m.dims, base[i],
+                       nzb[i], dc + base[i], dr + nzb[i], dd + base[i]);
+    if (hipGetLastError() != hipSuccess) {
+      H->err = "gather launch failed";
+      return SSQ_EDEVICE;
+    }
+  }
+  U->base = std::move(base);
+  U->N = N;
+  U->NNZ = NNZ;
+  U->colptr = dc;
+  U->rows = dr;
+  U->dims = dd;
+  return SSQ_OK;
+}
+
 static int reduce_blocks(Handle* H, const ssq_matrix* const* mats, int count, const ssq_params& p, ssq_pairs* outs,
                          ssq_stats* st) {
   for (int i = 0; i < count; i++) std::memset(&outs[i], 0, sizeof(ssq_pairs));
@@ -3583,6 +3648,73 @@ extern "C" int ssq_batch_reduce(ssq_handle* h, const ssq_matrix* const* mats, in
   h->h->err.clear();
   if (hipSetDevice(h->h->eng.device()) != hipSuccess) return SSQ_EDEVICE;
   return reduce_blocks(h->h, mats, count, *p, outs, stats);
+}
+
+extern "C" int ssq_matrix_to_device(ssq_handle* h, const ssq_matrix* m, ssq_device_matrix* out) {
+  if (!h || !m || !out) return SSQ_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  Handle* H = h->h;
+  H->err.clear();
+  if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  // through the handle's input buffers, where the matrix is checked on the device
+  const ssq_matrix* one[1] = {m};
+  Uploaded U;
+  int rc = upload_blocks(H, one, 1, &U);
+  if (rc) return rc;
+  void *cp = nullptr, *rw = nullptr, *dm = nullptr;
+  if (hipMalloc(&cp, sizeof(int64_t) * (U.N + 1)) != hipSuccess ||
+      hipMalloc(&rw, sizeof(int32_t) * std::max<int64_t>(U.NNZ, 1)) != hipSuccess ||
+      hipMalloc(&dm, sizeof(int32_t) * std::max<int64_t>(U.N, 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    if (cp) (void)hipFree(cp);
+    if (rw) (void)hipFree(rw);
+    H->err = "device allocation failed";
+    return SSQ_ENOMEM;
+  }
+  if ((rc = H->eng.copy(cp, U.colptr, sizeof(int64_t) * (U.N + 1))) ||
+      (rc = H->eng.copy(rw, U.rows, sizeof(int32_t) * U.NNZ)) ||
+      (rc = H->eng.copy(dm, U.dims, sizeof(int32_t) * U.N)) || hipStreamSynchronize(H->eng.stream()) != hipSuccess) {
+    (void)hipFree(cp);
+    (void)hipFree(rw);
+    (void)hipFree(dm);
+    H->err = "device copy failed";
+    return rc ? rc : SSQ_EDEVICE;
+  }
+  out->n = U.N;
+  out->nnz = U.NNZ;
+  out->col_ptr = (const int64_t*)cp;
+  out->rows = (const int32_t*)rw;
+  out->dims = (const int32_t*)dm;
+  return SSQ_OK;
+}
+
+extern "C" int ssq_device_matrix_free(ssq_handle* h, ssq_device_matrix* m) {
+  if (!h || !m) return SSQ_EINVAL;
+  if (hipSetDevice(h->h->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  for (const void* p : {(const void*)m->col_ptr, (const void*)m->rows, (const void*)m->dims})
+    if (p) (void)hipFree(const_cast<void*>(p));
+  std::memset(m, 0, sizeof(*m));
+  return SSQ_OK;
+}
+
+extern "C" int ssq_batch_reduce_resident(ssq_handle* h, const ssq_device_matrix* mats, int count, const ssq_params* p,
+                                         ssq_pairs* outs, ssq_stats* stats) {
+  if (!h || !p || !outs || count < 0 || (count > 0 && !mats)) return SSQ_EINVAL;
+  Handle* H = h->h;
+  H->err.clear();
+  for (int i = 0; i < count; i++) std::memset(&outs[i], 0, sizeof(ssq_pairs));
+  if (hipSetDevice(H->eng.device()) != hipSuccess) return SSQ_EDEVICE;
+  Uploaded U;
+  int rc = gather_blocks(H, mats, count, &U);
+  if (rc) return rc;
+  int64_t np = 0, ne = 0;
+  int64_t* dp = (int64_t*)H->out_pairs.p;
+  int64_t* de = (int64_t*)H->out_ess.p;
+  if ((rc = H->eng.reduce_device(U.N, U.NNZ, U.colptr, U.rows, U.dims, *p, dp, &np, stats, de, &ne))) {
+    H->err = H->eng.error();
+    return rc;
+  }
+  return split_pairs(H, dp, np, de, ne, U.base.data(), count, outs);
 }
 
 extern "C" int ssq_device_alloc(ssq_handle* h, size_t bytes, void** dptr) {

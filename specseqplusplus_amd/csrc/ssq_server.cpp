@@ -1,7 +1,7 @@
 // Continuous-batching request server on one GPU (new; no reference counterpart).
 //
-// Requests (boundary matrices, or point clouds that are turned into Vietoris-Rips matrices)
-// enter one queue.  Each worker owns an engine handle with its own HIP stream.  A worker
+// Requests (boundary matrices, point clouds that are turned into Vietoris-Rips matrices, or
+// matrices registered once and kept resident in device memory) wait in one queue per kind.  Each worker owns an engine handle with its own HIP stream.  A worker
 // that is free takes the oldest requests that fit both max_batch and the shared budget of
 // estimated device bytes, reduces them as one block-diagonal batch, publishes each result
 // and returns the budget.  With several workers, a batch is formed and launched while other
@@ -37,6 +37,7 @@ struct Request {
   int pdim = 0, maxdim = 0;
   float r = 0;
   bool is_points = false;
+  int64_t mid = 0;  // registered matrix (0: none)
   int64_t est_bytes = 0;
   int64_t est_cells = 0;
 };
@@ -64,6 +65,14 @@ void estimate_points(const float* pts, int64_t npts, int pdim, float r, int maxd
   *bytes = estimate_bytes(*cells, 2 * e + 3 * t + 4 * tets);
 }
 
+// A matrix registered for resident requests: its device copy and the requests that use it.
+struct Resident {
+  ssq_device_matrix dm{};
+  int64_t refs = 0;  // queued or running requests
+  bool closing = false;
+  int64_t est_bytes = 0;
+};
+
 // Host checks of a submitted matrix (rows are range-checked on the device): offsets start at
 // 0, never decrease and end at nnz.
 bool matrix_ok(const ssq_matrix* m) {
@@ -72,6 +81,20 @@ bool matrix_ok(const ssq_matrix* m) {
   if (m->col_ptr[0] != 0 || m->col_ptr[m->n] != m->nnz) return false;
   for (int64_t c = 0; c < m->n; c++)
     if (m->col_ptr[c + 1] < m->col_ptr[c]) return false;
+  return true;
+}
+
+// The device-side checks, on the host, for a matrix that is registered once: every column's
+// rows strictly increasing and in [0, n).
+bool rows_ok(const ssq_matrix* m) {
+  for (int64_t c = 0; c < m->n; c++) {
+    int64_t prev = -1;
+    for (int64_t k = m->col_ptr[c]; k < m->col_ptr[c + 1]; k++) {
+      const int64_t r = m->rows[k];
+      if (r <= prev || r >= m->n) return false;
+      prev = r;
+    }
+  }
   return true;
 }
 
@@ -153,7 +176,9 @@ struct ssq_server {
   ssq_server_config cfg;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Request*> queue[2];  // [0] matrices, [1] point clouds
+  std::deque<Request*> queue[3];  // [0] matrices, [1] point clouds, [2] registered matrices
+  std::unordered_map<int64_t, Resident> resident;
+  int64_t next_mid = 1;
   std::deque<Request*> sizing;    // point clouds whose cells are being counted
   std::condition_variable cv_size;
   std::vector<std::thread> sizers;
@@ -163,6 +188,7 @@ struct ssq_server {
   int64_t in_flight_bytes = 0;
   int64_t outstanding = 0;
   int ready = 0;  // workers that finished their warm-up
+  ssq_server_stats stats{};
   bool stop = false;
   int init_status = SSQ_OK;
 
@@ -183,15 +209,18 @@ struct ssq_server {
     cv_done.notify_all();
     for (;;) {
       std::vector<Request*> batch;
+      std::vector<ssq_device_matrix> dms;  // registered batches: the resident matrices
       int batch_kind = 0;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return (stop && sizing.empty()) || !queue[0].empty() || !queue[1].empty(); });
-        if (queue[0].empty() && queue[1].empty()) break;  // stopping, nothing left to size
-        // one kind per batch (each has its own parameters): the queue whose head is older
-        int kind = queue[0].empty() ? 1
-                   : queue[1].empty() ? 0
-                   : (queue[1].front()->t_submit < queue[0].front()->t_submit ? 1 : 0);
+        auto pending = [&] { return !queue[0].empty() || !queue[1].empty() || !queue[2].empty(); };
+        cv_work.wait(lk, [&] { return (stop && sizing.empty()) || pending(); });
+        if (!pending()) break;  // stopping, nothing left to size
+        // one kind per batch (each has its own parameters and input path): the queue whose
+        // head is oldest
+        int kind = -1;
+        for (int k = 0; k < 3; k++)
+          if (!queue[k].empty() && (kind < 0 || queue[k].front()->t_submit < queue[kind].front()->t_submit)) kind = k;
         batch_kind = kind;
         auto& Q = queue[kind];
         // admission: oldest first while it fits the batch size and the device budget; the
@@ -208,17 +237,23 @@ struct ssq_server {
           Q.pop_front();
           in_flight_bytes += q->est_bytes;
           batch.push_back(q);
+          if (kind == 2) dms.push_back(resident[q->mid].dm);  // held by the request's reference
         }
         if (batch.empty()) {
           // budget full: wait for another worker to return budget
+          dms.clear();
           cv_done.wait(lk);
           continue;
         }
       }
+      const auto t_start = Clock::now();
+      int64_t batch_cells = 0;
+      for (Request* q : batch) batch_cells += q->est_cells;
       std::vector<ssq_matrix> mats(batch.size());
       std::vector<int> st(batch.size(), SSQ_OK);
       const bool gpu_points = batch_kind == 1 && batch[0]->maxdim <= 2;
-      for (size_t i = 0; i < batch.size() && !gpu_points; i++) {
+      const bool registered = batch_kind == 2;
+      for (size_t i = 0; i < batch.size() && !gpu_points && !registered; i++) {
         Request* q = batch[i];
         if (q->is_points) {
           // dimension 3: the Rips matrix is built on the host
@@ -254,6 +289,10 @@ struct ssq_server {
         brc = h ? ssq_batch_reduce_points(h, (int)batch.size(), pp.data(), pn.data(), batch[0]->pdim, pr.data(),
                                           batch[0]->maxdim, &cfg.points_params, outs.data(), nullptr)
                 : SSQ_EDEVICE;
+      } else if (registered) {
+        // gathered from their resident copies on this worker's stream
+        brc = h ? ssq_batch_reduce_resident(h, dms.data(), (int)dms.size(), &cfg.params, outs.data(), nullptr)
+                : SSQ_EDEVICE;
       } else if (!ptrs.empty()) {
         const ssq_params* bp = batch_kind ? &cfg.points_params : &cfg.params;
         brc = h ? ssq_batch_reduce(h, ptrs.data(), (int)ptrs.size(), bp, outs.data(), nullptr) : SSQ_EDEVICE;
@@ -271,6 +310,8 @@ struct ssq_server {
             const float* p1 = q->pts.data();
             rc1 = ssq_batch_reduce_points(h, 1, &p1, &q->npts, q->pdim, &q->r, q->maxdim, &cfg.points_params,
                                           &outs[k], nullptr);
+          } else if (registered) {
+            rc1 = ssq_batch_reduce_resident(h, &dms[i], 1, &cfg.params, &outs[k], nullptr);
           } else {
             const ssq_params* bp = batch_kind ? &cfg.points_params : &cfg.params;
             rc1 = ssq_batch_reduce(h, &ptrs[k], 1, bp, &outs[k], nullptr);
@@ -304,12 +345,19 @@ struct ssq_server {
           R.ess.assign(outs[k].essential, outs[k].essential + outs[k].ne);
           ssq_pairs_free(&outs[k]);
         }
+        const double service = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+        stats.batches[batch_kind]++;
+        stats.requests[batch_kind] += (int64_t)batch.size();
+        stats.cells[batch_kind] += batch_cells;
+        stats.service_ms[batch_kind] += service;
+        stats.max_service_ms[batch_kind] = std::max(stats.max_service_ms[batch_kind], service);
         for (Request* q : batch) {
           in_flight_bytes -= q->est_bytes;
           outstanding--;
+          if (q->mid) resident[q->mid].refs--;
         }
       }
-      for (size_t i = 0; i < batch.size() && !gpu_points; i++)
+      for (size_t i = 0; i < batch.size() && !gpu_points && !registered; i++)
         if (batch[i]->is_points && st[i] == SSQ_OK) ssq_matrix_free(&mats[i]);
       for (Request* q : batch) delete q;
       cv_done.notify_all();
@@ -331,7 +379,17 @@ struct ssq_server {
       }
       q->id = next_id++;
       q->t_submit = Clock::now();
-      (q->is_points ? sizing : queue[0]).push_back(q);
+      if (q->mid) {
+        auto it = resident.find(q->mid);
+        if (it == resident.end() || it->second.closing) {
+          delete q;
+          return SSQ_EINVAL;
+        }
+        it->second.refs++;
+        q->est_bytes = it->second.est_bytes;
+        q->est_cells = it->second.dm.n;
+      }
+      (q->is_points ? sizing : q->mid ? queue[2] : queue[0]).push_back(q);
       outstanding++;
       *ticket = q->id;
     }
@@ -414,6 +472,66 @@ extern "C" int ssq_server_submit_points(ssq_server* s, const float* pts, int64_t
   return s->submit(q, ticket);
 }
 
+extern "C" int ssq_server_register_matrix(ssq_server* s, const ssq_matrix* m, int64_t* matrix_id) {
+  if (!s || !m || !matrix_id) return SSQ_EINVAL;
+  *matrix_id = 0;
+  if (!matrix_ok(m) || !rows_ok(m)) return SSQ_EFORMAT;
+  if (hipSetDevice(s->cfg.device) != hipSuccess) return SSQ_EDEVICE;
+  // synchronous copies on the null stream; the workers' streams do not wait for it
+  Resident R;
+  void *cp = nullptr, *rw = nullptr, *dm = nullptr;
+  const size_t bc = sizeof(int64_t) * (size_t)(m->n + 1), br = sizeof(int32_t) * (size_t)m->nnz,
+               bd = sizeof(int32_t) * (size_t)m->n;
+  bool ok = hipMalloc(&cp, bc) == hipSuccess && hipMalloc(&rw, std::max<size_t>(br, 4)) == hipSuccess &&
+            hipMalloc(&dm, std::max<size_t>(bd, 4)) == hipSuccess;
+  if (ok && m->n == 0) {
+    const int64_t zero = 0;
+    ok = hipMemcpy(cp, &zero, sizeof(zero), hipMemcpyHostToDevice) == hipSuccess;
+  } else if (ok) {
+    ok = hipMemcpy(cp, m->col_ptr, bc, hipMemcpyHostToDevice) == hipSuccess &&
+         (br == 0 || hipMemcpy(rw, m->rows, br, hipMemcpyHostToDevice) == hipSuccess) &&
+         hipMemcpy(dm, m->dims, bd, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    for (void* p : {cp, rw, dm})
+      if (p) (void)hipFree(p);
+    return SSQ_ENOMEM;
+  }
+  R.dm = ssq_device_matrix{m->n, m->nnz, (const int64_t*)cp, (const int32_t*)rw, (const int32_t*)dm};
+  R.est_bytes = estimate_bytes(m->n, m->nnz);
+  std::lock_guard<std::mutex> g(s->mu);
+  *matrix_id = s->next_mid++;
+  s->resident[*matrix_id] = R;
+  return SSQ_OK;
+}
+
+extern "C" int ssq_server_submit_registered(ssq_server* s, int64_t matrix_id, int64_t* ticket) {
+  if (!s || !ticket || matrix_id <= 0) return SSQ_EINVAL;
+  Request* q = new (std::nothrow) Request();
+  if (!q) return SSQ_ENOMEM;
+  q->mid = matrix_id;
+  return s->submit(q, ticket);
+}
+
+extern "C" int ssq_server_unregister_matrix(ssq_server* s, int64_t matrix_id) {
+  if (!s) return SSQ_EINVAL;
+  ssq_device_matrix dm;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->resident.find(matrix_id);
+    if (it == s->resident.end() || it->second.closing) return SSQ_EINVAL;
+    it->second.closing = true;  // no new requests on it
+    s->cv_done.wait(lk, [&] { return s->resident[matrix_id].refs == 0; });
+    dm = s->resident[matrix_id].dm;
+    s->resident.erase(matrix_id);
+  }
+  if (hipSetDevice(s->cfg.device) != hipSuccess) return SSQ_EDEVICE;
+  for (const void* p : {(const void*)dm.col_ptr, (const void*)dm.rows, (const void*)dm.dims})
+    (void)hipFree(const_cast<void*>(p));
+  return SSQ_OK;
+}
+
 extern "C" int ssq_server_poll(ssq_server* s, int64_t ticket, ssq_pairs* out, double* latency_ms) {
   if (!s || !out) return SSQ_EINVAL;
   std::memset(out, 0, sizeof(*out));
@@ -437,6 +555,13 @@ extern "C" int ssq_server_poll(ssq_server* s, int64_t ticket, ssq_pairs* out, do
   return SSQ_OK;
 }
 
+extern "C" int ssq_server_get_stats(ssq_server* s, ssq_server_stats* out) {
+  if (!s || !out) return SSQ_EINVAL;
+  std::lock_guard<std::mutex> g(s->mu);
+  *out = s->stats;
+  return SSQ_OK;
+}
+
 extern "C" int ssq_server_wait_all(ssq_server* s) {
   if (!s) return SSQ_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);
@@ -457,6 +582,11 @@ extern "C" int ssq_server_destroy(ssq_server* s) {
   for (auto& t : s->workers) t.join();
   for (auto& Q : s->queue)
     for (Request* q : Q) delete q;
+  if (!s->resident.empty() && hipSetDevice(s->cfg.device) == hipSuccess)
+    for (auto& kv : s->resident)
+      for (const void* p : {(const void*)kv.second.dm.col_ptr, (const void*)kv.second.dm.rows,
+                            (const void*)kv.second.dm.dims})
+        (void)hipFree(const_cast<void*>(p));
   delete s;
   return SSQ_OK;
 }

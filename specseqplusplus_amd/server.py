@@ -27,6 +27,22 @@ class Server:
         L.check(L.lib().ssq_server_submit_matrix(self._s, ctypes.byref(ms), ctypes.byref(t)))
         return t.value
 
+    def register_matrix(self, m):
+        """Keep m resident in device memory for repeated requests; returns its id."""
+        mid = ctypes.c_int64()
+        ms = m.c_struct()
+        L.check(L.lib().ssq_server_register_matrix(self._s, ctypes.byref(ms), ctypes.byref(mid)), "register")
+        return mid.value
+
+    def submit_registered(self, matrix_id):
+        t = ctypes.c_int64()
+        L.check(L.lib().ssq_server_submit_registered(self._s, int(matrix_id), ctypes.byref(t)))
+        return t.value
+
+    def unregister_matrix(self, matrix_id):
+        """Waits for the matrix's outstanding requests, then frees its device copy."""
+        L.check(L.lib().ssq_server_unregister_matrix(self._s, int(matrix_id)), "unregister")
+
     def submit_points(self, pts, r, maxdim=2):
         pts = np.ascontiguousarray(pts, dtype=np.float32)
         t = ctypes.c_int64()
@@ -47,6 +63,19 @@ class Server:
         ess = np.ctypeslib.as_array(pr.essential, shape=(max(pr.ne, 1),))[:pr.ne].copy()
         L.lib().ssq_pairs_free(ctypes.byref(pr))
         return pairs, ess, lat.value
+
+    def stats(self):
+        """Per batch kind (host matrices, point clouds, registered matrices): batches, requests,
+        cells, mean and maximum batch service time in ms."""
+        st = L.ssq_server_stats()
+        L.check(L.lib().ssq_server_get_stats(self._s, ctypes.byref(st)), "stats")
+        out = {}
+        for k, name in enumerate(("matrices", "clouds", "registered")):
+            b = st.batches[k]
+            out[name] = {"batches": b, "requests": st.requests[k], "cells": st.cells[k],
+                         "mean_service_ms": round(st.service_ms[k] / b, 3) if b else None,
+                         "max_service_ms": round(st.max_service_ms[k], 3)}
+        return out
 
     def wait_all(self):
         L.check(L.lib().ssq_server_wait_all(self._s), "wait_all")

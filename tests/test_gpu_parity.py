@@ -146,6 +146,43 @@ def test_batch_equals_individual(h):
         assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)))
 
 
+def test_resident_batch_equals_host_batch(h):
+    """ssq_batch_reduce_resident on matrices placed once with ssq_matrix_to_device: each
+    block's pairs and essentials equal the host batch's and the oracle's, with an empty matrix
+    between two others and one matrix used twice in the batch, in homology and cohomology."""
+    mats = []
+    for seed in range(9):
+        X = np.asarray(C.uniform_points(30 + 11 * seed, 3, 90 + seed), dtype=np.float32)
+        mats.append(S.vr_build(X, 0.3, 2))
+    mats.insert(4, S.Matrix(np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.int32)))
+    dms = [h.to_device(m) for m in mats]
+    try:
+        order = list(range(len(mats))) + [2]
+        for kw in ({}, {"dualize": 1}):
+            res = h.batch_reduce_resident([dms[i] for i in order], S.params(**kw))
+            host = h.batch_reduce([mats[i] for i in order], S.params(**kw))
+            for i, (pairs, ess), (hp, he) in zip(order, res, host):
+                m = mats[i]
+                assert np.array_equal(pairs, hp) and np.array_equal(ess, he), (i, kw)
+                assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims))), (i, kw)
+        assert h.batch_reduce_resident([]) == []
+    finally:
+        for dm in dms:
+            h.free_device_matrix(dm)
+
+
+def test_resident_placement_checks_the_matrix(h):
+    with pytest.raises(S.SSQError) as e:
+        h.to_device(_malformed("unsorted"))
+    assert e.value.status == _lib.SSQ_EFORMAT
+    good = as_product(c1_matrix())
+    dm = h.to_device(good)
+    try:
+        assert np.array_equal(h.batch_reduce_resident([dm])[0][0], O.twist_pairs(c1_matrix()))
+    finally:
+        h.free_device_matrix(dm)
+
+
 def test_device_resident_entry_point(h):
     mo = c1_matrix()
     dm = h.upload(as_product(mo))

@@ -108,3 +108,51 @@ def test_server_dense_cloud_under_a_small_budget():
             m = S.vr_build(X, r, 2)
             pairs = srv.poll(t)[0]
             assert np.array_equal(pairs, O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims), dual=True)), len(X)
+
+
+def test_server_registered_matrices():
+    """Matrices registered once and kept in device memory, submitted by id among host matrices
+    and point clouds: every result equals the oracle's; a malformed matrix is refused at
+    registration, an unknown or unregistered id at submission, and the ids still registered
+    stay usable after another is unregistered."""
+    rng = np.random.default_rng(12)
+    mats = [S.vr_build(rng.random((40 + 9 * i, 3)).astype(np.float32), 0.3, 2) for i in range(6)]
+    refs = [O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims)) for m in mats]
+    clouds = [rng.random((30 + i, 3)).astype(np.float32) for i in range(6)]
+    with Server(streams=2, max_batch=6, p=S.params(), points_p=S.params(dualize=1, ss_block_size=1 << 30)) as srv:
+        bad = S.Matrix(mats[0].colptr.copy(), mats[0].rows.copy(), mats[0].dims.copy())
+        bad.rows[-1] = bad.n + 3
+        with pytest.raises(L.SSQError) as e:
+            srv.register_matrix(bad)
+        assert e.value.status == L.SSQ_EFORMAT
+        ids = [srv.register_matrix(m) for m in mats]
+        reqs = []
+        for k in range(60):
+            j = int(rng.integers(0, len(mats)))
+            if k % 3 == 0:
+                reqs.append(("reg", j, srv.submit_registered(ids[j])))
+            elif k % 3 == 1:
+                reqs.append(("mat", j, srv.submit_matrix(mats[j])))
+            else:
+                reqs.append(("pts", j, srv.submit_points(clouds[j], 0.3, 2)))
+        srv.wait_all()
+        for kind, j, t in reqs:
+            pairs = srv.poll(t)[0]
+            if kind == "pts":
+                m = S.vr_build(clouds[j], 0.3, 2)
+                ref = O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims), dual=True)
+            else:
+                ref = refs[j]
+            assert np.array_equal(pairs, ref), (kind, j)
+        with pytest.raises(L.SSQError):
+            srv.submit_registered(10 ** 6)
+        t = srv.submit_registered(ids[1])
+        srv.unregister_matrix(ids[1])  # waits for that request
+        assert np.array_equal(srv.poll(t)[0], refs[1])
+        with pytest.raises(L.SSQError):
+            srv.submit_registered(ids[1])
+        with pytest.raises(L.SSQError):
+            srv.unregister_matrix(ids[1])
+        t = srv.submit_registered(ids[2])
+        srv.wait_all()
+        assert np.array_equal(srv.poll(t)[0], refs[2])

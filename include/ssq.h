@@ -216,6 +216,24 @@ int ssq_server_destroy(ssq_server* s);
  * (Flag_complex_edge_collapser.h:503-524).  threads <= 0: all host threads.  No GPU. */
 int ssq_collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kept, int threads);
 
+/* ---- Alpha complexes (replaces phat-highclear/addons/alpha_3.cpp, CGAL Delaunay_triangulation_3
+ * with an exact kernel).  Host only.  npts x 3 float64 points; duplicates count once, vertex ids
+ * follow each distinct point's first occurrence.  The Delaunay triangulation uses exact
+ * predicates and symbolic perturbation (unique, independent of input order); values follow
+ * alpha_3.cpp:260-352 (tetrahedra: squared circumradius; triangles and edges: the value of an
+ * attached coface, else their own squared radius; vertices 0); cells with value <= max_value
+ * (max_value < 0: all) in (value, dimension, vertex ids) order as alpha_3.cpp writes them
+ * (:354-356), or with dim_major = 1 by dimension first, then value and vertex ids, as the
+ * reference's data/Alpha files are stored.  values (may be NULL) receives each cell's value
+ * (malloc'ed, n doubles; release with ssq_free).  SSQ_EINVAL when the points do not span 3-D
+ * or their coordinates span more than 120 bits of one binary grid. */
+int ssq_alpha_build(const double* pts, int64_t npts, double max_value, int dim_major, ssq_matrix* out,
+                    double** values);
+void ssq_free(void* p);
+/* The Delaunay tetrahedra alone: 4 ascending vertex ids each (ids as above); *ntets receives
+ * the count, SSQ_ENOMEM when it exceeds cap. */
+int ssq_delaunay_3(const double* pts, int64_t npts, int32_t* tets, int64_t cap, int64_t* ntets);
+
 /* ---- Inputs (phat-highclear/addons/rips.cpp:49-87 semantics) ------------------------- */
 /* Vietoris-Rips boundary matrix of npts points in R^pdim: edges with float32 distance
  * <= r, cliques up to dimension maxdim, ordered by (filtration, dim, lexicographic vertex

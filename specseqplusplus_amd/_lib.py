@@ -111,6 +111,11 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
     "ssq_vr_build_gpu": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_int, ctypes.POINTER(ssq_matrix)]),
+    "ssq_alpha_build": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int64, ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(ssq_matrix), ctypes.POINTER(ctypes.POINTER(ctypes.c_double))]),
+    "ssq_free": (None, [ctypes.c_void_p]),
+    "ssq_delaunay_3": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.c_int64, i64p]),
     "ssq_collapse_edges": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                           ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]),
     "ssq_batch_reduce_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,

@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <random>
 #include <string>
@@ -124,6 +125,40 @@ int main(int argc, char** argv) {
   }
   ssq_matrix bad{};
   EXPECT(ssq_vr_build(nullptr, 5, 3, 0.1f, 2, &bad) != SSQ_OK);
+  // alpha complexes: random points, a cospherical grid with duplicates, and refused inputs
+  for (int t = 0; t < 3; t++) {
+    std::vector<double> P;
+    if (t < 2) {
+      for (int i = 0; i < 3 * (60 + 90 * t); i++) P.push_back(U(rng));
+    } else {
+      for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++)
+          for (int z = 0; z < 4; z++) P.insert(P.end(), {0.5 * x, 0.5 * y, 0.5 * z});
+      P.insert(P.end(), P.begin(), P.begin() + 12);  // four duplicates
+    }
+    const int64_t n = (int64_t)P.size() / 3;
+    ssq_matrix a{};
+    double* vals = nullptr;
+    EXPECT(ssq_alpha_build(P.data(), n, t == 1 ? -1.0 : 0.05, t & 1, &a, &vals) == SSQ_OK);
+    for (int64_t c = 0; c < a.n; c++)
+      for (int64_t k = a.col_ptr[c]; k < a.col_ptr[c + 1]; k++) EXPECT(a.rows[k] < c);
+    for (int64_t c = 1; c < a.n && !(t & 1); c++) EXPECT(vals[c] >= vals[c - 1]);
+    std::vector<int32_t> tets(4 * 8 * (size_t)n + 64);
+    int64_t nt = 0;
+    EXPECT(ssq_delaunay_3(P.data(), n, tets.data(), (int64_t)tets.size() / 4, &nt) == SSQ_OK && nt > 0);
+    ssq_free(vals);
+    ssq_matrix_free(&a);
+  }
+  {
+    std::vector<double> flat;  // coplanar points: refused
+    for (int i = 0; i < 20; i++) flat.insert(flat.end(), {U(rng), U(rng), 0.0});
+    ssq_matrix a{};
+    EXPECT(ssq_alpha_build(flat.data(), 20, 1.0, 0, &a, nullptr) == SSQ_EINVAL);
+    std::vector<double> nan_pts(12, 0.25);
+    nan_pts[5] = std::nan("");
+    EXPECT(ssq_alpha_build(nan_pts.data(), 4, 1.0, 0, &a, nullptr) == SSQ_EINVAL);
+    EXPECT(ssq_alpha_build(nan_pts.data(), 3, 1.0, 0, &a, nullptr) == SSQ_EINVAL);
+  }
   std::printf("asan driver: %d failed checks\n", fails);
   return fails ? 1 : 0;
 }

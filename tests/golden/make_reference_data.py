@@ -6,7 +6,8 @@ Each matrix is stored compactly: column lengths and dimensions (uint8), and the 
 column as (column index - first row), then successive differences (uint32), the three arrays
 concatenated and LZMA-compressed.  Beside it: n, nnz and the SHA-256 of the oracle's pairs
 (twist reduction, primal, int64 (birth, death) rows in birth order), the golden value the GPU
-tests compare against.
+tests compare against.  Also written: indoor01_28k_points.npz, the point cloud of the
+data/Alpha matrices (point-net/indoor01_28k.txt), for the alpha generator's tests.
 
     python tests/golden/make_reference_data.py [/root/reference]
 """
@@ -68,6 +69,9 @@ def main():
         print(name, m.n, len(m.rows), len(pairs), len(blob))
     np.savez(os.path.join(HERE, "reference_data.npz"), index=np.frombuffer(json.dumps(index).encode(), np.uint8),
              **blobs)
+    # the point cloud the data/Alpha matrices were built from (point-net/indoor01_28k.txt)
+    pts = np.loadtxt(os.path.join(ref, "point-net", "indoor01_28k.txt"), dtype=np.float64, ndmin=2)
+    np.savez_compressed(os.path.join(HERE, "indoor01_28k_points.npz"), points=pts)
 
 
 if __name__ == "__main__":

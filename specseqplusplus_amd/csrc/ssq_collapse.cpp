@@ -10,7 +10,12 @@
 //  2. Cascades (set_edge_critical, :326-359).  For each edge that the main loop finds
 //     critical, in index order, the non-critical edges of its 3-cliques are re-tested in
 //     decreasing index order with the critical set of that moment.  This part is
-//     inherently sequential and runs on one thread.
+//     inherently sequential and runs on one thread.  A re-test is first decided from the
+//     edge's last witness: the valid edges only grow, so the common neighbours of an edge
+//     only grow and its witness's valid neighbourhood only grows; every new common neighbour
+//     comes from an edge turned critical, whose 3-clique pass records it for the re-tested
+//     edge.  If the witness is adjacent by a valid edge to every recorded vertex that is now a
+//     valid common neighbour, the edge is still dominated; otherwise the full test runs.
 //
 // The domination predicate (:224-303) is: with I the common valid neighbours of (a,b),
 // the edge is dominated iff |I| == 1 or some w in I has I inside its closed valid
@@ -95,13 +100,24 @@ inline bool covers(const Graph& g, int32_t w, const int32_t* I, int64_t m, Valid
   return true;
 }
 
+// id of edge (u, v), or -1 (neighbours are ascending)
+inline int32_t edge_id(const Graph& g, int32_t u, int32_t v) {
+  const int32_t* b = g.nbr.data() + g.off[u];
+  const int32_t* e = g.nbr.data() + g.off[u + 1];
+  const int32_t* p = std::lower_bound(b, e, v);
+  return (p != e && *p == v) ? g.eid[p - g.nbr.data()] : -1;
+}
+
 // The predicate of edge_is_dominated: |I| == 1 or some w in I covers I.  *witness is tried
 // first and receives the dominating vertex found (the answer does not depend on the order
 // in which candidates are tried).
 template <typename Valid>
 bool dominated(const Graph& g, int32_t e, Valid valid, int32_t* I, int32_t* witness) {
   const int64_t m = common_e(g, e, valid, I, nullptr, nullptr);
-  if (m <= 1) return m == 1;
+  if (m <= 1) {
+    if (m == 1) *witness = I[0];
+    return m == 1;
+  }
   const int32_t hint = *witness;
   if (hint >= 0 && std::binary_search(I, I + m, hint) && covers(g, hint, I, m, valid)) return true;
   for (int64_t c = 0; c < m; c++) {
@@ -242,21 +258,57 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
   std::vector<int32_t> I((size_t)maxdeg), ea((size_t)maxdeg), eb((size_t)maxdeg);
   std::vector<int32_t> stamp((size_t)ne, -1);
   std::priority_queue<int32_t> heap;
+  // vertices that may have become common neighbours of a pushed edge since its last test: a
+  // list per edge in a pool that lives for one cascade (every pushed edge is popped in it)
+  std::vector<int32_t> head((size_t)ne, -1), head_sid((size_t)ne, -1), pool_next, pool_x;
+  auto add_cand = [&](int32_t e, int32_t x, int32_t sid) {
+    if (head_sid[e] != sid) {
+      head_sid[e] = sid;
+      head[e] = -1;
+    }
+    pool_next.push_back(head[e]);
+    pool_x.push_back(x);
+    head[e] = (int32_t)pool_x.size() - 1;
+  };
   auto three_clique = [&](int32_t thresh, int32_t e, int32_t sid) {
     const uint8_t* crit = g.crit.data();
     int64_t m = common_e(g, e, [thresh, crit](int32_t x) { return x < thresh || crit[x]; }, I.data(), ea.data(),
                          eb.data());
+    const int32_t p = g.eu[e], q = g.ev[e];
     for (int64_t k = 0; k < m; k++) {
-      int32_t e1 = ea[k], e2 = eb[k];
-      if (!crit[e1] && e1 < thresh && stamp[e1] != sid) { stamp[e1] = sid; heap.push(e1); }
-      if (!crit[e2] && e2 < thresh && stamp[e2] != sid) { stamp[e2] = sid; heap.push(e2); }
+      const int32_t e1 = ea[k], e2 = eb[k];  // e1 = (p, x) gains q, e2 = (q, x) gains p
+      if (!crit[e1] && e1 < thresh) {
+        add_cand(e1, q, sid);
+        if (stamp[e1] != sid) { stamp[e1] = sid; heap.push(e1); }
+      }
+      if (!crit[e2] && e2 < thresh) {
+        add_cand(e2, p, sid);
+        if (stamp[e2] != sid) { stamp[e2] = sid; heap.push(e2); }
+      }
     }
+  };
+  // still dominated by its last witness: every recorded vertex is the witness, or joined to it
+  // by a valid edge, or not a valid common neighbour
+  auto witness_holds = [&](int32_t cur, int32_t sid) {
+    const int32_t w = witness[cur];
+    if (w < 0) return false;
+    const uint8_t* crit = g.crit.data();
+    auto valid = [cur, crit](int32_t x) { return x >= 0 && (x < cur || crit[x]); };
+    const int32_t a = g.eu[cur], b = g.ev[cur];
+    for (int32_t t = head_sid[cur] == sid ? head[cur] : -1; t >= 0; t = pool_next[t]) {
+      const int32_t x = pool_x[t];
+      if (x == w || valid(edge_id(g, w, x))) continue;
+      if (valid(edge_id(g, a, x)) && valid(edge_id(g, b, x))) return false;
+    }
+    return true;
   };
   std::fill(kept, kept + ne, 0);
   g.crit[0] = 1;
   kept[0] = 1;
   for (int32_t idx = 1; idx < (int32_t)ne; idx++) {
     if (!main_crit[idx]) continue;
+    pool_next.clear();
+    pool_x.clear();
     g.crit[idx] = 1;
     kept[idx] = 1;
     three_clique(idx, idx, idx);
@@ -264,6 +316,7 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
       int32_t cur = heap.top();
       heap.pop();
       if (g.crit[cur]) continue;
+      if (witness_holds(cur, idx)) continue;
       const uint8_t* crit = g.crit.data();
       if (dominated(g, cur, [cur, crit](int32_t x) { return x < cur || crit[x]; }, I.data(), &witness[cur]))
         continue;

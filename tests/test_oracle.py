@@ -192,3 +192,34 @@ def test_cohomology_planted_mistake_is_caught():
     n = m.n
     wrong = np.stack([n - 1 - p[:, 0], n - 1 - p[:, 1]], 1)
     assert not np.array_equal(wrong[np.argsort(wrong[:, 0])], O.rank_pairs(m))
+
+
+def _edges_of(m):
+    e = np.where(np.asarray(m.dims) == 1)[0]
+    cp, rows = np.asarray(m.colptr), np.asarray(m.rows)
+    return rows[cp[e]].astype(np.int32), rows[cp[e] + 1].astype(np.int32)
+
+
+def test_collapse_on_the_large_rips_edge_list():
+    """The full edge list of the 4096-rotation Rips complex (r = 1.0, 163,560 edges): the
+    library's host collapse (parallel main loop, cascades decided from the last witness where
+    they can be) keeps exactly the oracle's edges, and the reference's GUDHI collapser's when
+    oracle/_ref is built."""
+    import specseqplusplus_amd as S
+    from specseqplusplus_amd import configs as CF
+    u, v = _edges_of(S.vr_build(CF.c2_points(), 1.0, 2))
+    assert len(u) == 163560
+    kept = O.collapse_edges(u, v)
+    assert np.array_equal(S.collapse_edges(u, v), kept)
+    assert np.array_equal(S.collapse_edges(u, v, threads=1), kept)
+    if O.ref_available():
+        assert np.array_equal(O.ref_collapse_edges(u, v), kept)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_library_collapse_equals_oracle(seed):
+    import specseqplusplus_amd as S
+    rng = np.random.default_rng(600 + seed)
+    X = rng.random((60 + 15 * seed, 2 + seed % 3)).astype(np.float32)
+    u, v = _edges_of(S.vr_build(X, 0.2 + 0.02 * seed, 1))
+    assert np.array_equal(S.collapse_edges(u, v), O.collapse_edges(u, v))

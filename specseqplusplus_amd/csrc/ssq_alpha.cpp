@@ -346,10 +346,16 @@ struct Delaunay {
     return (uint32_t)rng;
   }
 
-  // a cell in conflict with p, by a visibility walk from `start`
+  // a cell in conflict with p, by a visibility walk from `start` (facets tried from a random
+  // one, which ends with probability 1 in any triangulation; past a step budget, a scan)
   int locate(int start, int p) {
     int c = start;
-    for (;;) {
+    for (int64_t steps = 0;; steps++) {
+      if (steps > 64 + 4 * (int64_t)cells.size()) {
+        for (int k = 0; k < (int)cells.size(); k++)
+          if (alive[k] && in_conflict(k, p)) return k;
+        return start;  // not reached: every point conflicts with some cell
+      }
       const Cell& C = cells[c];
       const int i = index_of(C, kInf);
       if (i >= 0) {
@@ -566,7 +572,8 @@ bool prepare(const double* pts, int64_t n, Geom& G) {
     int e;
     const double m = std::frexp(v, &e);
     const int64_t M = (int64_t)std::ldexp(m, 53);  // exact
-    G.X[i] = (i128)M << ((e - 53) - lo);
+    const u128 mag = (u128)(M < 0 ? -M : M) << ((e - 53) - lo);  // below 2^120
+    G.X[i] = M < 0 ? -(i128)mag : (i128)mag;
   }
   return true;
 }

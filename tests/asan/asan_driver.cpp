@@ -129,7 +129,7 @@ int main(int argc, char** argv) {
   for (int t = 0; t < 3; t++) {
     std::vector<double> P;
     if (t < 2) {
-      for (int i = 0; i < 3 * (60 + 90 * t); i++) P.push_back(U(rng));
+      for (int i = 0; i < 3 * (60 + 90 * t); i++) P.push_back(t ? U(rng) - 0.5 : U(rng));  // signs both ways
     } else {
       for (int x = 0; x < 3; x++)
         for (int y = 0; y < 3; y++)

@@ -36,7 +36,6 @@
 // t is its latest edge (written with the own entries), that of edge row (a, b), a < b, is
 // vertex b; a column leftmost in some row is negative (Lemma L in the dual).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -50,6 +49,7 @@
 
 #include "ssq_internal.h"
 #include "ssq_kernels.h"
+#include "ssq_sort.h"
 
 namespace ssq {
 
@@ -717,12 +717,6 @@ __global__ void k_rc_pack(int C, const int64_t* slot, const int64_t* off, const 
 
 __global__ void k_rc_set64(int64_t* p, int64_t v) { *p = v; }
 
-// out[n] = out[n - 1] + in[n - 1]: the total after an exclusive scan
-template <typename In>
-__global__ void k_rc_total(const In* in, int64_t n, int64_t* out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = out[n - 1] + (int64_t)in[n - 1];
-}
-
 __global__ void k_rc_dims(int64_t nv, int64_t nc, int32_t* dims) {
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
     dims[c] = c < nv ? 2 : 1;  // dual dimension: vertices 2, edges 1
@@ -826,12 +820,9 @@ int scan64(RipsCo* B, hipStream_t s, const In* in, int64_t n, int64_t* out, std:
     hipLaunchKernelGGL(k_rc_set64, dim3(1), dim3(1), 0, s, out, (int64_t)0);
     return SSQ_OK;
   }
-  size_t tb = 0;
-  RCCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, s));
   uint8_t* tmp;
-  RCGROW(temp, uint8_t, tb, tmp);
-  RCCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, (int)n, s));
-  hipLaunchKernelGGL(k_rc_total<In>, dim3(1), dim3(1), 0, s, in, n, out);
+  RCGROW(temp, uint8_t, ssq_sort::scan_bytes(n), tmp);
+  RCCHK((ssq_sort::scan<In, int64_t, false>(in, n, out, tmp, s)));  // out[n] = the total
   return SSQ_OK;
 }
 
@@ -1029,10 +1020,10 @@ static int rc_group(RipsCo* B, const RipsCo* S, PrebuiltTarget* eng, hipStream_t
   while ((1ll << gbits) < G) gbits++;
   if (NE > 0) {
     size_t tb2 = 0;
-    RCCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, keys, keys2, vals, vals2, (int)NE, 0, 31 + gbits, s));
+    RCCHK(ssq_sort::sort_pairs<uint64_t>(nullptr, tb2, keys, keys2, vals, vals2, NE, 31 + gbits, s));
     uint8_t* tmp;
     RCGROW(temp, uint8_t, tb2, tmp);
-    RCCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb2, keys, keys2, vals, vals2, (int)NE, 0, 31 + gbits, s));
+    RCCHK(ssq_sort::sort_pairs<uint64_t>(tmp, tb2, keys, keys2, vals, vals2, NE, 31 + gbits, s));
   }
 
   // ---- engine buffers (rows are written straight into its pool) ----
@@ -1065,11 +1056,9 @@ static int rc_group(RipsCo* B, const RipsCo* S, PrebuiltTarget* eng, hipStream_t
   if (NE > 0) {
     hipLaunchKernelGGL(k_rc_edges, dim3(ge), dim3(kBlock), 0, s, q, (const int32_t*)vals2, (const int32_t*)erow,
                        (const int32_t*)enbr, (const uint64_t*)keys2, suv, ecl, R, rstart, leftone, negflag);
-    size_t tb2 = 0;
-    RCCHK(hipcub::DeviceScan::InclusiveSum(nullptr, tb2, rstart, runincl, (int)NE, s));
     uint8_t* tmp;
-    RCGROW(temp, uint8_t, tb2, tmp);
-    RCCHK(hipcub::DeviceScan::InclusiveSum(tmp, tb2, rstart, runincl, (int)NE, s));
+    RCGROW(temp, uint8_t, ssq_sort::scan_bytes(NE), tmp);
+    RCCHK((ssq_sort::scan<int32_t, int32_t, true>(rstart, NE, runincl, tmp, s)));
     hipLaunchKernelGGL(k_rc_runpos, dim3(ge), dim3(kBlock), 0, s, NE, (const int32_t*)rstart,
                        (const int32_t*)runincl, runpos);
     hipLaunchKernelGGL(k_rc_runs, dim3(ge), dim3(kBlock), 0, s, NE, (const int32_t*)runincl, (const int32_t*)runpos,
@@ -1158,10 +1147,10 @@ static int rc_group(RipsCo* B, const RipsCo* S, PrebuiltTarget* eng, hipStream_t
     int rbits = 1;
     while ((1ll << rbits) < nrun) rbits++;
     size_t tb2 = 0;
-    RCCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, tk, tk2, tv, tv2, (int)ntie, 0, 36 + rbits, s));
+    RCCHK(ssq_sort::sort_pairs<uint64_t>(nullptr, tb2, tk, tk2, tv, tv2, ntie, 36 + rbits, s));
     uint8_t* tmp;
     RCGROW(temp, uint8_t, tb2, tmp);
-    RCCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb2, tk, tk2, tv, tv2, (int)ntie, 0, 36 + rbits, s));
+    RCCHK(ssq_sort::sort_pairs<uint64_t>(tmp, tb2, tk, tk2, tv, tv2, ntie, 36 + rbits, s));
     hipLaunchKernelGGL(k_rc_tieapply, dim3(grid_of(ntie)), dim3(kBlock), 0, s, ntie, (const uint64_t*)tk2,
                        (const int32_t*)tv2, (const int64_t*)rto, (const int64_t*)rr0, remap);
   }

@@ -39,11 +39,25 @@ struct Graph {
   // of (a, x) and (b, x) -- the edge's triangles.  The valid common neighbours of a test are
   // a filter of this list, not a merge of two adjacency lists.  Empty when too large.
   std::vector<int64_t> toff;
-  std::unique_ptr<int32_t[]> tx, tea, teb;  // edge e's triangles: [toff[e], toff[e] + tcnt[e])
+  int32_t *tx = nullptr, *tea = nullptr, *teb = nullptr;  // edge e's triangles: [toff[e], toff[e] + tcnt[e])
   std::vector<int32_t> tcnt;
   std::vector<int32_t> eu, ev;
 };
+// Storage of the triangle lists (three arrays of int32), grown as needed and never shrunk.
+struct TriBuffers {
+  std::unique_ptr<int32_t[]> x, a, b;
+  size_t cap = 0;
+  void reserve(size_t n) {
+    if (n <= cap) return;
+    const size_t want = n + n / 4;
+    x.reset(new int32_t[want]);  // filled on all threads; a vector would zero them on one
+    a.reset(new int32_t[want]);
+    b.reset(new int32_t[want]);
+    cap = want;
+  }
+};
 constexpr int64_t kTriListMax = int64_t(1) << 28;  // entries (12 bytes each)
+constexpr int64_t kBitsetVertices = 16384;          // neighbourhood bit sets up to 32 MB
 
 template <typename Valid>
 int64_t common(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* out, int32_t* ea,
@@ -210,21 +224,73 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
   for (int64_t a = 0; a < nv; a++) maxdeg = std::max(maxdeg, g.off[a + 1] - g.off[a]);
   g.eu = eu;
   g.ev = ev;
-  {  // triangle lists of every edge, in one pass: edge e gets min(deg a, deg b) slots
+  {  // triangle lists of every edge
     std::vector<int64_t> cap((size_t)ne + 1, 0);
-    for (int64_t e = 0; e < ne; e++)
-      cap[e + 1] = cap[e] + std::min(g.off[eu[e] + 1] - g.off[eu[e]], g.off[ev[e] + 1] - g.off[ev[e]]);
-    if (cap[ne] <= kTriListMax) {
-      // filled below on all threads; a vector would zero them first on one thread
-      g.tx.reset(new int32_t[(size_t)cap[ne]]);
-      g.tea.reset(new int32_t[(size_t)cap[ne]]);
-      g.teb.reset(new int32_t[(size_t)cap[ne]]);
-      g.tcnt.resize((size_t)ne);
-      parallel_edges([&](int64_t e) {
-        const int64_t o = cap[e];
-        g.tcnt[e] = (int32_t)common(g, eu[e], ev[e], [](int32_t) { return true; }, g.tx.get() + o,
-                                    g.tea.get() + o, g.teb.get() + o);
+    g.tcnt.assign((size_t)ne, 0);
+    const int64_t W = (nv + 63) / 64;
+    std::vector<uint64_t> nb;
+    std::vector<int32_t> pre;
+    if (nv <= kBitsetVertices) {
+      // neighbourhoods as bit sets: an edge's common neighbours are the set bits of one AND
+      // per word, in increasing order, and a neighbour's position in a sorted adjacency list
+      // is the popcount of the set bits below it (a prefix count per word plus one word)
+      nb.assign((size_t)(nv * W), 0);
+      pre.assign((size_t)(nv * W), 0);
+      parallel_range(nv, 16, [&](int64_t a) {
+        uint64_t* row = nb.data() + a * W;
+        for (int64_t p = g.off[a]; p < g.off[a + 1]; p++) row[g.nbr[p] >> 6] |= 1ull << (g.nbr[p] & 63);
+        int32_t run = 0;
+        for (int64_t w = 0; w < W; w++) {
+          pre[(size_t)(a * W + w)] = run;
+          run += __builtin_popcountll(row[w]);
+        }
       });
+      parallel_edges([&](int64_t e) {  // exact counts first
+        const uint64_t *ra = nb.data() + (int64_t)eu[e] * W, *rb = nb.data() + (int64_t)ev[e] * W;
+        int32_t k = 0;
+        for (int64_t w = 0; w < W; w++) k += __builtin_popcountll(ra[w] & rb[w]);
+        g.tcnt[e] = k;
+      });
+      for (int64_t e = 0; e < ne; e++) cap[e + 1] = cap[e] + g.tcnt[e];
+    } else {  // merges: edge e gets min(deg a, deg b) slots
+      for (int64_t e = 0; e < ne; e++)
+        cap[e + 1] = cap[e] + std::min(g.off[eu[e] + 1] - g.off[eu[e]], g.off[ev[e] + 1] - g.off[ev[e]]);
+    }
+    if (cap[ne] <= kTriListMax) {
+      // kept by the calling thread between calls: fresh pages cost more than filling them
+      thread_local TriBuffers keep;
+      keep.reserve((size_t)cap[ne]);
+      g.tx = keep.x.get();
+      g.tea = keep.a.get();
+      g.teb = keep.b.get();
+      if (nv <= kBitsetVertices) {
+        parallel_edges([&](int64_t e) {
+          const int32_t a = eu[e], b = ev[e];
+          const uint64_t *ra = nb.data() + (int64_t)a * W, *rb = nb.data() + (int64_t)b * W;
+          const int32_t *pa = pre.data() + (int64_t)a * W, *pb = pre.data() + (int64_t)b * W;
+          int32_t* tx = g.tx + cap[e];
+          int32_t* ta = g.tea + cap[e];
+          int32_t* tb = g.teb + cap[e];
+          int32_t k = 0;
+          for (int64_t w = 0; w < W; w++) {
+            uint64_t m = ra[w] & rb[w];
+            while (m) {
+              const int bit = __builtin_ctzll(m);
+              const uint64_t below = (1ull << bit) - 1;
+              tx[k] = (int32_t)(w * 64 + bit);
+              ta[k] = g.eid[g.off[a] + pa[w] + __builtin_popcountll(ra[w] & below)];
+              tb[k] = g.eid[g.off[b] + pb[w] + __builtin_popcountll(rb[w] & below)];
+              k++;
+              m &= m - 1;
+            }
+          }
+        });
+      } else {
+        parallel_edges([&](int64_t e) {
+          const int64_t o = cap[e];
+          g.tcnt[e] = (int32_t)common(g, eu[e], ev[e], [](int32_t) { return true; }, g.tx + o, g.tea + o, g.teb + o);
+        });
+      }
       g.toff.swap(cap);
     }
   }

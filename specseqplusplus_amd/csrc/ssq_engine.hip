@@ -1344,18 +1344,21 @@ __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t
         }
         if (tid == 0) atomicMin(&sh_minw, (b_first - lo) >> 5);
         __syncthreads();
-        // new low and the row below it, in the window
+        // new low and the row below it, in the window; consecutive additions use alternate
+        // slots of red[], so an addition's slot is rewritten only two barriers after every
+        // thread has read it
+        const int rb = (int)(chain & 1u) * 2;
         if (tid < kWave) {
           int32_t t1, t2;
           window_top2(bits, ((l - lo) >> 5) - ww0, lo + (ww0 << 5), &t1, &t2);
           if (tid == 0) {
-            red[0] = t1;
-            red[1] = t2;
+            red[rb] = t1;
+            red[rb + 1] = t2;
           }
         }
         __syncthreads();
-        int32_t newl = red[0];
-        a2 = red[1];
+        int32_t newl = red[rb];
+        a2 = red[rb + 1];
         if (newl < 0 && ww0 > 0) {
           // the window is empty: wait for the global toggles, take the tail's highest row and
           // move the window onto it
@@ -1408,7 +1411,6 @@ __global__ void __launch_bounds__(kLongThreads) k_add_huge(Dev d, int r, int32_t
         max_len = max(max_len, (uint32_t)b_len);
         chain += 1;
         l = newl;
-        __syncthreads();  // red[] read by all before the next search writes it
       } else {
         // tiled merge by wave 0 into the other of two slots: the column's own and one scratch
         // slot, swapped after each addition; the scratch slot is replaced (twice the need)

@@ -42,6 +42,11 @@ struct Graph {
   int32_t *tx = nullptr, *tea = nullptr, *teb = nullptr;  // edge e's triangles: [toff[e], toff[e] + tcnt[e])
   std::vector<int32_t> tcnt;
   std::vector<int32_t> eu, ev;
+  // with at most kBitsetVertices vertices: each neighbourhood as a bit set (W words per
+  // vertex) and per word the number of neighbours below it
+  std::vector<uint64_t> nb;
+  std::vector<int32_t> pre;
+  int64_t W = 0;
 };
 // Storage of the triangle lists (three arrays of int32), grown as needed and never shrunk.
 struct TriBuffers {
@@ -101,9 +106,31 @@ int64_t common(const Graph& g, int32_t a, int32_t b, Valid valid, int32_t* out, 
   return k;
 }
 
+// id of edge (u, v), or -1 (neighbours are ascending)
+inline int32_t edge_id(const Graph& g, int32_t u, int32_t v) {
+  if (g.W) {
+    const uint64_t word = g.nb[(size_t)(u * g.W + (v >> 6))];
+    const uint64_t bit = 1ull << (v & 63);
+    if (!(word & bit)) return -1;
+    return g.eid[g.off[u] + g.pre[(size_t)(u * g.W + (v >> 6))] + __builtin_popcountll(word & (bit - 1))];
+  }
+  const int32_t* b = g.nbr.data() + g.off[u];
+  const int32_t* e = g.nbr.data() + g.off[u + 1];
+  const int32_t* p = std::lower_bound(b, e, v);
+  return (p != e && *p == v) ? g.eid[p - g.nbr.data()] : -1;
+}
+
 // does w's closed valid neighbourhood contain all of I[0:m)?
 template <typename Valid>
 inline bool covers(const Graph& g, int32_t w, const int32_t* I, int64_t m, Valid valid) {
+  if (g.W) {
+    for (int64_t t = 0; t < m; t++) {
+      if (I[t] == w) continue;
+      const int32_t e = edge_id(g, w, I[t]);
+      if (e < 0 || !valid(e)) return false;
+    }
+    return true;
+  }
   int64_t p = g.off[w], pe = g.off[w + 1];
   for (int64_t t = 0; t < m; t++) {
     const int32_t x = I[t];
@@ -112,14 +139,6 @@ inline bool covers(const Graph& g, int32_t w, const int32_t* I, int64_t m, Valid
     if (!(p < pe && g.nbr[p] == x && valid(g.eid[p]))) return false;
   }
   return true;
-}
-
-// id of edge (u, v), or -1 (neighbours are ascending)
-inline int32_t edge_id(const Graph& g, int32_t u, int32_t v) {
-  const int32_t* b = g.nbr.data() + g.off[u];
-  const int32_t* e = g.nbr.data() + g.off[u + 1];
-  const int32_t* p = std::lower_bound(b, e, v);
-  return (p != e && *p == v) ? g.eid[p - g.nbr.data()] : -1;
 }
 
 // The predicate of edge_is_dominated: |I| == 1 or some w in I covers I.  *witness is tried
@@ -228,9 +247,10 @@ void collapse_edges(int64_t ne, const int32_t* u, const int32_t* v, uint8_t* kep
     std::vector<int64_t> cap((size_t)ne + 1, 0);
     g.tcnt.assign((size_t)ne, 0);
     const int64_t W = (nv + 63) / 64;
-    std::vector<uint64_t> nb;
-    std::vector<int32_t> pre;
+    std::vector<uint64_t>& nb = g.nb;
+    std::vector<int32_t>& pre = g.pre;
     if (nv <= kBitsetVertices) {
+      g.W = W;
       // neighbourhoods as bit sets: an edge's common neighbours are the set bits of one AND
       // per word, in increasing order, and a neighbour's position in a sorted adjacency list
       // is the popcount of the set bits below it (a prefix count per word plus one word)

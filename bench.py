@@ -175,13 +175,14 @@ def parity_leg(h, pts, rs, per, args):
         [int(i) for i in np.random.default_rng(1).permutation(len(pts)) if i != np.argmax([len(X) for X in pts])]
     it = iter(order)
     done, busy, exact, cells, oracle_bytes = 0, 0.0, True, 0, 0.0
+    oracle_adds, oracle_chain, sample = 0.0, 0.0, []
     t_start = time.perf_counter()
 
     def work(i, m):
         mo = O.Matrix(m.colptr, m.rows, m.dims)
         tt = time.perf_counter()
         ref, ost = O.twist_pairs(mo, dual=True, stats=True)
-        return i, ref, time.perf_counter() - tt, m.n, ost["bytes"]
+        return i, ref, time.perf_counter() - tt, m.n, ost
 
     with ThreadPoolExecutor(workers) as ex:
         futs = set()
@@ -195,20 +196,43 @@ def parity_leg(h, pts, rs, per, args):
                 break
             fin, futs = wait(futs, return_when=FIRST_COMPLETED)
             for f in fin:
-                i, ref, dt, n, ob = f.result()
+                i, ref, dt, n, ost = f.result()
                 busy += dt
                 done += 1
                 cells += n
-                oracle_bytes += ob
+                oracle_bytes += ost["bytes"]
+                oracle_adds += ost["additions"]
+                oracle_chain = max(oracle_chain, ost["max_chain"])
+                sample.append(i)
                 exact &= bool(ref.shape == per[i].shape and np.array_equal(ref, per[i]))
     wall = time.perf_counter() - t_start
     log(f"parity: {done} clouds ({cells} cells) bit-exact={exact}; oracle {done / busy:.2f}/s per core")
+    # the engine's additions on the same clouds, one batched call: no schedule of the engine
+    # may add more columns than the sequential reduction does (the GPU cannot inflate B_alg)
+    import specseqplusplus_amd as S
+    sp = [pts[i] for i in sample]
+    pb = np.concatenate([[0], np.cumsum([len(X) for X in sp])]).astype(np.int64)
+    d_sp = h.device_array(np.concatenate(sp).astype(np.float32))
+    srad = np.array([rs[i] for i in sample], np.float32)
+    se, _ = h.rips_count(d_sp, pb, srad)
+    scap = int(pb[-1] + se.sum())
+    d_sp_pairs = h.device_empty(2 * scap, np.int64)
+    _, est = h.batch_reduce_points_device(d_sp, pb, srad, d_sp_pairs, scap, S.params(**BATCH_PARAMS), stats=True)
+    for a in (d_sp, d_sp_pairs):
+        a.free()
+    adds = {"oracle_additions": oracle_adds, "engine_additions": float(est["n_additions"]),
+            "engine_over_oracle": round(float(est["n_additions"]) / max(oracle_adds, 1.0), 4),
+            "oracle_max_chain": oracle_chain, "engine_max_chain": float(est["max_chain"]),
+            "oracle_bytes": oracle_bytes}
+    log(f"additions on the sample: engine {adds['engine_additions']:.0f}, oracle {oracle_adds:.0f} "
+        f"(ratio {adds['engine_over_oracle']}); longest chain engine {adds['engine_max_chain']:.0f}, "
+        f"oracle {oracle_chain:.0f}")
     cpu = {"value": round(done / wall, 3), "unit": "diagrams/s", "cores": workers, "kind": "port",
            "single_core_value": round(done / busy, 3),
            "sample": f"{done} clouds of this stream (the largest, then random ones), {cells} cells; oracle twist "
                      f"reduction in cohomology (anti-transpose, bit-tree pivot column, clearing), one cloud per "
                      f"thread on {workers} threads, {wall:.1f} s wall (matrix builds not timed)"}
-    return {"checked": done, "cells": cells, "bit_exact": exact}, cpu
+    return {"checked": done, "cells": cells, "bit_exact": exact, "additions_on_sample": adds}, cpu
 
 
 def plain_leg(pts, rs, args):

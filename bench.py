@@ -175,7 +175,7 @@ def parity_leg(h, pts, rs, per, args):
         [int(i) for i in np.random.default_rng(1).permutation(len(pts)) if i != np.argmax([len(X) for X in pts])]
     it = iter(order)
     done, busy, exact, cells, oracle_bytes = 0, 0.0, True, 0, 0.0
-    oracle_adds, oracle_chain, sample = 0.0, 0.0, []
+    oracle_adds, oracle_chain, oracle_depth, sample = 0.0, 0.0, 0.0, []
     t_start = time.perf_counter()
 
     def work(i, m):
@@ -203,6 +203,7 @@ def parity_leg(h, pts, rs, per, args):
                 oracle_bytes += ost["bytes"]
                 oracle_adds += ost["additions"]
                 oracle_chain = max(oracle_chain, ost["max_chain"])
+                oracle_depth = max(oracle_depth, ost["depth"])
                 sample.append(i)
                 exact &= bool(ref.shape == per[i].shape and np.array_equal(ref, per[i]))
     wall = time.perf_counter() - t_start
@@ -223,6 +224,7 @@ def parity_leg(h, pts, rs, per, args):
     adds = {"oracle_additions": oracle_adds, "engine_additions": float(est["n_additions"]),
             "engine_over_oracle": round(float(est["n_additions"]) / max(oracle_adds, 1.0), 4),
             "oracle_max_chain": oracle_chain, "engine_max_chain": float(est["max_chain"]),
+            "oracle_dependency_depth": oracle_depth,
             "oracle_bytes": oracle_bytes}
     log(f"additions on the sample: engine {adds['engine_additions']:.0f}, oracle {oracle_adds:.0f} "
         f"(ratio {adds['engine_over_oracle']}); longest chain engine {adds['engine_max_chain']:.0f}, "

@@ -126,13 +126,14 @@ def twist_pairs(m, skip=None, stats=False, dual=False):
     """Persistence pairs (np x 2 int64, sorted by birth) by twist reduction; dual=True reduces
     the anti-transpose (cohomology) and maps the pairs back."""
     out = np.zeros((max(m.n, 1), 2), dtype=np.int64)
-    st = np.zeros(3, dtype=np.float64)
+    st = np.zeros(4, dtype=np.float64)
     s, sp = _skip(m, skip)
     f = lib().or_twist_reduce_dual if dual else lib().or_twist_reduce
     k = f(m.n, _p(m.colptr, _i64p), _p(m.rows, _i32p), _p(m.dims, _i32p), sp, _p(out, _i64p), _p(st, _dp))
     pairs = out[:k].copy()
     if stats:
-        return pairs, {"additions": float(st[0]), "bytes": float(st[1]), "max_chain": float(st[2])}
+        return pairs, {"additions": float(st[0]), "bytes": float(st[1]), "max_chain": float(st[2]),
+                       "depth": float(st[3])}
     return pairs
 
 

@@ -261,18 +261,23 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
   int64_t acap = 1 << 20, aused = 0;
   int32_t* arena = (int32_t*)malloc(sizeof(int32_t) * (size_t)acap);
   int32_t* ext = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
-  double adds = 0, bytes = 0, max_chain = 0;
+  double adds = 0, bytes = 0, max_chain = 0, critical = 0;
+  /* fin[c]: when column c would be final if every addition took one step and a column could
+   * add a pivot only once that pivot is final (unlimited parallelism, dimensions in turn) */
+  double* fin = (double*)calloc((size_t)(n ? n : 1), sizeof(double));
   bitree t;
   bt_init(&t, n);
   for (int32_t d = maxd; d >= 1; d--) {
+    double dim_critical = 0;
     for (int64_t j = 0; j < n; j++) {
       if (dims[j] != d || cleared[j] || (skip && skip[j])) continue;
       for (int64_t k = colptr[j]; k < colptr[j + 1]; k++) bt_toggle(&t, rows[k]);
       int64_t low;
-      double col_adds = 0;
+      double col_adds = 0, tj = 0;
       while ((low = bt_max(&t)) >= 0 && pivot_of[low] >= 0) {
         col_adds += 1;
         int32_t p = pivot_of[low];
+        tj = (fin[p] > tj ? fin[p] : tj) + 1;
         int64_t a = t.count;
         const int32_t* col = arena + roff[p];
         for (int32_t k = 0; k < rlen[p]; k++) bt_toggle(&t, col[k]);
@@ -280,6 +285,8 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
         bytes += 4.0 * (double)(a + rlen[p] + t.count) + 24.0;
       }
       if (col_adds > max_chain) max_chain = col_adds;
+      fin[j] = tj;
+      if (tj > dim_critical) dim_critical = tj;
       if (low < 0) continue;
       /* store the reduced column (descending extraction, then reverse) */
       int64_t len = 0;
@@ -299,8 +306,10 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
       pivot_of[low] = (int32_t)j;
       cleared[low] = 1; /* twist clearing: column low is positive, reduces to zero */
     }
+    critical += dim_critical;
   }
   bt_free(&t);
+  free(fin);
   int32_t* dob = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
   for (int64_t i = 0; i < n; i++) dob[i] = pivot_of[i];
   int64_t np = emit_pairs(n, dob, pairs_out);
@@ -308,6 +317,7 @@ int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
     stats_out[0] = adds;
     stats_out[1] = bytes;
     stats_out[2] = max_chain;
+    stats_out[3] = critical;
   }
   free(dob);
   free(pivot_of);

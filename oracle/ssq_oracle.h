@@ -61,8 +61,11 @@ int or_save_pairs(const char* path, int binary, int64_t np, const int64_t* pairs
 
 /* Twist reduction with clearing. skip[c]!=0 empties column c beforehand (sub-complex).
  * pairs_out must hold 2*n int64. Returns the number of pairs (sorted by birth).
- * stats_out (may be NULL) receives {additions, algorithmic bytes, most additions made by one
- * column} where one addition a <- a xor b costs 4*(|a|+|b|+|a xor b|)+24 bytes. */
+ * stats_out (may be NULL, else 4 doubles) receives {additions, algorithmic bytes, most
+ * additions made by one column, dependency depth} where one addition a <- a xor b costs
+ * 4*(|a|+|b|+|a xor b|)+24 bytes, and the dependency depth is the sum over dimensions of the
+ * longest chain of additions when each addition takes one step and may start once the pivot
+ * it adds is final (the time of an unlimited-parallel reduction, in additions). */
 int64_t or_twist_reduce(int64_t n, const int64_t* colptr, const int32_t* rows,
                         const int32_t* dims, const uint8_t* skip, int64_t* pairs_out,
                         double* stats_out);

@@ -223,3 +223,20 @@ def test_library_collapse_equals_oracle(seed):
     X = rng.random((60 + 15 * seed, 2 + seed % 3)).astype(np.float32)
     u, v = _edges_of(S.vr_build(X, 0.2 + 0.02 * seed, 1))
     assert np.array_equal(S.collapse_edges(u, v), O.collapse_edges(u, v))
+
+
+def test_dependency_depth_bounds():
+    """The oracle's dependency depth (longest chain of additions when each waits only for the
+    pivot it adds) lies between the longest single chain and the number of additions, and is
+    0 when nothing is added."""
+    import specseqplusplus_amd as S
+    for seed in range(6):
+        X = np.random.default_rng(70 + seed).random((40 + 20 * seed, 3)).astype(np.float32)
+        m = S.vr_build(X, 0.35, 2)
+        for dual in (False, True):
+            _, st = O.twist_pairs(O.Matrix(m.colptr, m.rows, m.dims), stats=True, dual=dual)
+            assert st["max_chain"] <= st["depth"] <= st["additions"]
+    tri = O.Matrix(np.array([0, 0, 0, 0, 2, 4, 6, 9]), np.array([0, 1, 0, 2, 1, 2, 3, 4, 5], np.int32),
+                   np.array([0, 0, 0, 1, 1, 1, 2], np.int32))
+    _, st = O.twist_pairs(tri, stats=True)
+    assert st["additions"] == 0 and st["depth"] == 0

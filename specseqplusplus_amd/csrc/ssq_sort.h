@@ -194,8 +194,9 @@ hipError_t scan(const In* in, int64_t n, Out* out, void* scratch, hipStream_t s)
   return hipGetLastError();
 }
 
-// Stable sort of (kin, vin) by key bits [0, end_bit) into (kout, vout).  With tmp == nullptr,
-// only sets bytes (the scratch size); count < 2^32.
+// Stable sort of (kin, vin) by key into (kout, vout), for keys below 2^end_bit: the passes
+// cover whole 8-bit digits, ceil(end_bit / 8) of them, so any bits of a key up to the next
+// multiple of 8 take part.  With tmp == nullptr, only sets bytes (the scratch size); count < 2^32.
 template <typename Key>
 hipError_t sort_pairs(void* tmp, size_t& bytes, const Key* kin, Key* kout, const int32_t* vin, int32_t* vout,
                       int64_t count, int end_bit, hipStream_t s) {

@@ -24,6 +24,9 @@ import time
 from concurrent.futures import ThreadPoolExecutor
 
 os.environ.setdefault("SSQ_KERNEL_TIMING", "1")
+# sixteen hardware queues per process (HIP's default is 4), read when HIP starts: the batched
+# Rips path runs one stream per queue, and its groups' reductions overlap on them
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -34,6 +37,10 @@ from specseqplusplus_amd import workloads as W  # noqa: E402
 METRIC = "pairs bit-exact; diagrams/s + p50/p99: batched server vs plain per-complex GPU"
 HBM_PEAK_GBS = 8000.0
 BATCH_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=64, level2=1024)
+# Device bytes per group of clouds in the batched leg (SSQ_RIPS_GROUP_BYTES): sixteen groups in
+# flight, one per stream, fit the GPU's memory with the streams' device regions.  Fixed, so
+# that the one-stream roofline step and its rocprofv3 runs reduce the same groups.
+GROUP_BYTES = 9_000_000_000
 SERVED_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=16384, level1=64, level2=1024)
 SERVED_MATRIX_PARAMS = dict(dualize=0, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=1000,
                             level2=16000)
@@ -89,8 +96,16 @@ def roofline_record(st):
 # ---------------------------------------------------------------------------------------
 # legs
 # ---------------------------------------------------------------------------------------
+def rips_streams():
+    """Streams of the batched Rips path, by libssq's rule: SSQ_RIPS_STREAMS, else one per
+    hardware queue (GPU_MAX_HW_QUEUES, HIP's default 4), at most 16."""
+    n = int(os.environ.get("SSQ_RIPS_STREAMS") or os.environ.get("GPU_MAX_HW_QUEUES") or 4)
+    return max(1, min(16, n))
+
+
 def batched_leg(h, pts, rs, args):
-    """The headline: every cloud's pairs, points resident in HBM, four streams."""
+    """The headline: every cloud's pairs, points resident in HBM, one stream per hardware
+    queue."""
     import specseqplusplus_amd as S
     pbase = np.concatenate([[0], np.cumsum([len(X) for X in pts])]).astype(np.int64)
     d_pts = h.device_array(np.concatenate(pts).astype(np.float32))
@@ -99,6 +114,7 @@ def batched_leg(h, pts, rs, args):
     cap = int(pbase[-1] + edges.sum())
     d_pairs = h.device_empty(2 * cap, np.int64)
     p = S.params(**BATCH_PARAMS)
+    os.environ["SSQ_RIPS_GROUP_BYTES"] = str(GROUP_BYTES)
     log(f"workload: {len(pts)} clouds, {pbase[-1]} points, {edges.sum()} edges, {tris.sum()} triangles")
     for i in range(args.warmup):
         t = time.perf_counter()
@@ -122,6 +138,7 @@ def batched_leg(h, pts, rs, args):
     _, st1 = h.batch_reduce_points_device(d_pts, pbase, rad, d_pairs, cap, p, stats=True)
     one_stream_ms = (time.perf_counter() - t) * 1e3
     del os.environ["SSQ_RIPS_STREAMS"]
+    del os.environ["SSQ_RIPS_GROUP_BYTES"]
     for a in (d_pts, d_pairs):
         a.free()
     return {"ms_per_step": dt * 1e3 / args.steps, "diagrams_per_s": len(pts) * args.steps / dt, "stats": stats,
@@ -526,7 +543,9 @@ def main():
         "config": {"workload": f"batched: {args.complexes} Rips complexes, n log-uniform [{args.nmin},{args.nmax}], "
                                f"cube/circle/torus, r = 30th percentile, dim<=2; points in HBM -> pairs in HBM",
                    "points": int(B["pbase"][-1]), "edges": int(B["edges"].sum()), "triangles": int(B["tris"].sum()),
-                   "cells": N, "nnz": NNZ, "pairs": npairs, "streams": 4, "groups_per_step": int(last["n_pages"]),
+                   "cells": N, "nnz": NNZ, "pairs": npairs, "streams": rips_streams(),
+                   "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]), "group_bytes": GROUP_BYTES,
+                   "groups_per_step": int(last["n_pages"]),
                    "params": BATCH_PARAMS},
         "plain_gpu_diagrams_per_s": round(plain["plain_gpu"], 3) if plain else None,
         "batched_over_plain": round(B["diagrams_per_s"] / plain["plain_gpu"], 2) if plain else None,

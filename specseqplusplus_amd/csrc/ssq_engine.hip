@@ -3764,13 +3764,18 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
 }
 
 // Streams of the batched Rips path: the handle's setting (ssq::handle_set_rips_streams), else
-// SSQ_RIPS_STREAMS (1-4), else 4.  HIP places a stream on one of GPU_MAX_HW_QUEUES hardware
-// queues (4 by default), so at most 4: one group per hardware queue.
+// SSQ_RIPS_STREAMS, else one per hardware queue: HIP spreads a process's streams over
+// GPU_MAX_HW_QUEUES queues (4 unless set), and two streams on one queue run one after the
+// other.  At most kMaxRipsStreams.  Each stream reduces its own groups; a group's reduction
+// keeps few columns chasing at a time, so more groups in flight fill more of the GPU (with
+// the budget split over more streams, each group is smaller).
+constexpr int kMaxRipsStreams = 16;
 static int rips_streams(const Handle* H) {
   int n = 4;
+  if (const char* q = std::getenv("GPU_MAX_HW_QUEUES")) n = std::atoi(q);
   if (const char* e = std::getenv("SSQ_RIPS_STREAMS")) n = std::atoi(e);
   if (H->rips_streams > 0) n = H->rips_streams;
-  return std::max(1, std::min(4, n));
+  return std::max(1, std::min(kMaxRipsStreams, n));
 }
 
 static int rips_reduce(Handle* H, int count, const float* d_pts, const int64_t* pbase, int pdim, const float* radius,

@@ -744,7 +744,8 @@ struct RipsCo {
       run_rank0, remap, tkeys, tkeys2, tvals, tvals2, na, nb, dims, flag, pos, hv, he, err, pairs, gpts, gA, gdeg,
       gup, stage, slots;
   int64_t* host = nullptr;  // pinned scratch for read-backs
-  int64_t budget = 0;       // device bytes one group may use
+  int64_t budget = 0;       // device bytes the streams' groups may use together
+  int budget_streams = 0;   // the streams it was split over (a group gets budget / budget_streams)
   bool trace = std::getenv("SSQ_TRACE_GROUPS") != nullptr;  // per-group sizes and times on stderr
   std::vector<int64_t> h_pb, h_ab;
   std::vector<float> h_rad;
@@ -1283,15 +1284,22 @@ int ripsco_reduce(RipsCo* const* Bs, PrebuiltTarget* const* engs, int nstreams, 
   RCGROW(stage, int64_t, 2 * std::max<int64_t>(need, 1), stage);
   RCGROW(slots, int64_t, 2 * ((int64_t)count + 1), dslot);
   doff = dslot + count + 1;
-  if (mem_budget <= 0)  // test hook: a small budget forces many groups
+  if (mem_budget <= 0)  // a fixed group size (tests force many groups with a small one)
     if (const char* e = std::getenv("SSQ_RIPS_GROUP_BYTES")) mem_budget = std::atoll(e);
-  if (mem_budget <= 0) {  // fixed at the first call, so that every call groups alike
+  if (mem_budget <= 0) {
+    // fixed at the first call, group size included, so that every call groups alike: a later
+    // call on fewer streams still finds the other streams' engines holding their buffers
     if (B->budget <= 0) {
       size_t fr = 0, tot = 0;
       RCCHK(hipMemGetInfo(&fr, &tot));
       B->budget = (int64_t)((double)fr * 0.75) - (int64_t)nstreams * kHugeRegionBytes;
+      B->budget_streams = nstreams;
     }
-    mem_budget = B->budget / nstreams;  // the streams' groups are in memory together
+    // the groups in flight together; more streams than at the first call also take the extra
+    // streams' device regions out of the budget
+    const int extra = std::max(0, nstreams - B->budget_streams);
+    mem_budget = std::max<int64_t>(1, (B->budget - (int64_t)extra * kHugeRegionBytes) /
+                                          std::max(nstreams, B->budget_streams));
   }
   // Groups: clouds by decreasing triangle count, consecutive ones while the group's estimated
   // bytes fit the budget and its cells stay below kGroupCells.  The largest clouds (whose

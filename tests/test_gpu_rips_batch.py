@@ -91,6 +91,24 @@ def test_groups_split_by_memory_budget(h, monkeypatch, budget):
         assert np.array_equal(a, b) and np.array_equal(ae, be)
 
 
+@pytest.mark.parametrize("streams", ["2", "16"])
+def test_groups_on_many_streams(h, monkeypatch, streams):
+    """The groups spread over several streams (each with its own engine and device region),
+    one cloud per group: pairs are unchanged.  A handle of its own, so that the extra engines
+    are released at its end."""
+    clouds = mixed_clouds(11)
+    ref = h.batch_reduce_points(clouds, 2, S.params(**P))
+    monkeypatch.setenv("SSQ_RIPS_GROUP_BYTES", "1")
+    monkeypatch.setenv("SSQ_RIPS_STREAMS", streams)
+    h2 = S.Handle(0)
+    try:
+        got = h2.batch_reduce_points(clouds, 2, S.params(**P))
+    finally:
+        h2.close()
+    for (a, ae), (b, be) in zip(ref, got):
+        assert np.array_equal(a, b) and np.array_equal(ae, be)
+
+
 def test_device_resident_points_and_pairs(h):
     """Points resident in HBM, pairs written to HBM with per-cloud offsets; the counts entry
     gives each cloud's edges and triangles."""

@@ -24,9 +24,10 @@ import time
 from concurrent.futures import ThreadPoolExecutor
 
 os.environ.setdefault("SSQ_KERNEL_TIMING", "1")
-# sixteen hardware queues per process (HIP's default is 4), read when HIP starts: the batched
-# Rips path runs one stream per queue, and its groups' reductions overlap on them
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# sixteen hardware queues per process (HIP's default is 4; SSQ_BENCH_HW_QUEUES changes it), read
+# when HIP starts: the batched Rips path runs one stream per queue, and its groups' reductions
+# overlap on them
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("SSQ_BENCH_HW_QUEUES", "16")
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -37,9 +38,9 @@ from specseqplusplus_amd import workloads as W  # noqa: E402
 METRIC = "pairs bit-exact; diagrams/s + p50/p99: batched server vs plain per-complex GPU"
 HBM_PEAK_GBS = 8000.0
 BATCH_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=64, level2=1024)
-# Device bytes per group of clouds in the batched leg (SSQ_RIPS_GROUP_BYTES): sixteen groups in
-# flight, one per stream, fit the GPU's memory with the streams' device regions.  Fixed, so
-# that the one-stream roofline step and its rocprofv3 runs reduce the same groups.
+# Device bytes per group of clouds in the batched leg (SSQ_RIPS_GROUP_BYTES, unless set): sixteen
+# groups in flight, one per stream, fit the GPU's memory with the streams' device regions.
+# Fixed, so that the one-stream roofline step and its rocprofv3 runs reduce the same groups.
 GROUP_BYTES = 9_000_000_000
 SERVED_PARAMS = dict(dualize=1, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=16384, level1=64, level2=1024)
 SERVED_MATRIX_PARAMS = dict(dualize=0, model=0, ss_block_size=1 << 30, growth=1.05, queue_size=65536, level1=1000,
@@ -114,7 +115,8 @@ def batched_leg(h, pts, rs, args):
     cap = int(pbase[-1] + edges.sum())
     d_pairs = h.device_empty(2 * cap, np.int64)
     p = S.params(**BATCH_PARAMS)
-    os.environ["SSQ_RIPS_GROUP_BYTES"] = str(GROUP_BYTES)
+    os.environ.setdefault("SSQ_RIPS_GROUP_BYTES", str(GROUP_BYTES))
+    group_bytes = int(os.environ["SSQ_RIPS_GROUP_BYTES"])
     log(f"workload: {len(pts)} clouds, {pbase[-1]} points, {edges.sum()} edges, {tris.sum()} triangles")
     for i in range(args.warmup):
         t = time.perf_counter()
@@ -143,7 +145,7 @@ def batched_leg(h, pts, rs, args):
         a.free()
     return {"ms_per_step": dt * 1e3 / args.steps, "diagrams_per_s": len(pts) * args.steps / dt, "stats": stats,
             "per": per, "edges": edges, "tris": tris, "pbase": pbase, "one_stream_ms": one_stream_ms,
-            "one_stream_stats": st1}
+            "one_stream_stats": st1, "group_bytes": group_bytes}
 
 
 def structural_checks(pts, per, edges, tris):
@@ -544,7 +546,7 @@ def main():
                                f"cube/circle/torus, r = 30th percentile, dim<=2; points in HBM -> pairs in HBM",
                    "points": int(B["pbase"][-1]), "edges": int(B["edges"].sum()), "triangles": int(B["tris"].sum()),
                    "cells": N, "nnz": NNZ, "pairs": npairs, "streams": rips_streams(),
-                   "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]), "group_bytes": GROUP_BYTES,
+                   "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]), "group_bytes": B["group_bytes"],
                    "groups_per_step": int(last["n_pages"]),
                    "params": BATCH_PARAMS},
         "plain_gpu_diagrams_per_s": round(plain["plain_gpu"], 3) if plain else None,

@@ -3769,7 +3769,7 @@ extern "C" int ssq_device_synchronize(ssq_handle* h) {
 // other.  At most kMaxRipsStreams.  Each stream reduces its own groups; a group's reduction
 // keeps few columns chasing at a time, so more groups in flight fill more of the GPU (with
 // the budget split over more streams, each group is smaller).
-constexpr int kMaxRipsStreams = 16;
+constexpr int kMaxRipsStreams = 16;  // with 4 GiB of device region each, 24 ran out of memory
 static int rips_streams(const Handle* H) {
   int n = 4;
   if (const char* q = std::getenv("GPU_MAX_HW_QUEUES")) n = std::atoi(q);
